@@ -1,0 +1,171 @@
+"""Benchmark of the fused D3Q27 double-distribution Rayleigh-Benard step.
+
+Workload (default): one 512^3 cavity, Ra=1e7, Pr=0.71, walls T=+-0.5,
+conductive start with a seeded +-1e-3 perturbation, FP64 storage and
+arithmetic; W warm-up steps, then K timed steps.  A "step" is one fused
+collide-and-stream launch over all cells (f and g).
+
+Prints one JSON line: MLUPS (whole job), the roofline of the fused kernel
+(algorithmic bytes 2*27*2*8 = 864 B per cell-update in FP64, 216 B in 16-bit
+storage, divided by the kernel's average duration from HIP events recorded on
+the context's stream around the timed launches), and the CPU baseline (the
+FP64 OpenMP oracle on a bounded sample, rank 0 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+BYTES_PER_CELL = {"fp64": 864.0, "fp16": 216.0, "bf16": 216.0}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload_key: str):
+    """HBM bytes per launch of the fused kernel measured with rocprofv3 PMC
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), recorded
+    in profiles/traffic.txt as `key bytes_per_launch`.  None if absent."""
+    p = ROOT / "profiles" / "traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == workload_key:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(size: int, ra: float, budget_s: float):
+    """Time the FP64 oracle (OpenMP) on a bounded sample of the workload."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    o = oracle.OracleState(size, size, size, rayleigh=ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    o.init_conductive(2, 1.0e-3)
+    o.run(2)  # warm-up (page faults, thread pool)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 10000:
+            break
+    mlups = size ** 3 * steps / el / 1e6
+    return {
+        "value": round(mlups, 3),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle (C/OpenMP transcription of src/), {size}^3 cavity Ra={ra:g}, "
+                  f"{steps} steps timed after 2 warm-up steps, {el:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--ra", type=float, default=1.0e7)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp16", "bf16"])
+    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
+              "note (DESIGN.md), not an implemented path. Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from lbm_ddf_amd import Cavity, bytes_required, make_params
+
+    L = args.size
+    n = L ** 3
+    prec = args.precision
+    need = bytes_required(make_params(L, L, L, precision=prec))
+    log(f"cavity {L}^3 {prec}: {need / 1e9:.1f} GB device memory")
+    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
+    c.init_conductive(2, 1.0e-3)
+    c.synchronize()
+    log(f"warm-up {args.warmup} steps")
+    t = time.perf_counter()
+    c.step(args.warmup)
+    c.synchronize()
+    log(f"warm-up done in {time.perf_counter() - t:.1f} s; timing {args.steps} steps")
+
+    c.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = c.step_timed(args.steps)  # events on the context stream around the launches
+    c.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed region {wall:.2f} s (events {kernel_ms / 1e3:.2f} s)")
+
+    d = c.reduce()
+    if d["nonfinite"] != 0:
+        log("WARNING: non-finite values in the final state")
+    c.close()
+
+    mlups = n * args.steps / wall / 1e6
+    avg_launch_s = kernel_ms / 1e3 / args.steps
+    achieved = BYTES_PER_CELL[prec] * n / avg_launch_s / 1e9
+    key = f"fused_{prec}_{L}"
+    traffic = traffic_from_profiles(key)
+
+    out = {
+        "metric": "MLUPS (fused D3Q27 f+g step, one cavity)",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else f"{prec} storage, f32 arithmetic",
+        "data": "synthetic: conductive start with seeded +-1e-3 perturbation",
+        "config": {
+            "workload": f"Rayleigh-Benard cavity {L}x{L}x{L}, Ra={args.ra:g}, Pr=0.71, walls +-0.5, "
+                        f"{prec} populations, fused collide+stream of f and g",
+            "cells": n,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL[prec] * n,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        },
+        "physics": {"nu_bulk": d["nu_bulk"], "nu_top": d["nu_top"], "nu_bottom": d["nu_bottom"],
+                    "mass": d["mass"], "kinetic_energy": d["kinetic_energy"],
+                    "step": d["step"]},
+    }
+    if not args.no_cpu:
+        log(f"CPU baseline: oracle on {args.cpu_size}^3 for ~{args.cpu_seconds:.0f} s")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.ra, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,172 @@
+/*
+ * lbm_ddf.h -- C ABI of the MI355X-native D3Q27 double-distribution
+ * Rayleigh-Benard step (library: lbm_ddf_amd/_lib/liblbm_ddf.so).
+ *
+ * This header is the drop-in boundary for the single-GPU time step of
+ * LBM_DDF.  The reference has no library: its interface is the header
+ * src/rb3d.h (kernel prototypes :49-62, host functions :47,64-68), global
+ * device pointers (src/globals.cu:28-30) and compile-time parameters
+ * (src/parameters.h:5-31).  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - every function returns 0 on success and a negative LBM_E* code on
+ *     failure; lbm_last_error() gives the message.  The library never calls
+ *     exit() (the reference's CHECK_CUDA_ERROR exits, src/rb3d.h:71-79);
+ *   - an opaque context replaces the global pointers and __constant__
+ *     symbols; it owns its device memory and one HIP stream;
+ *   - parameters are runtime values per context;
+ *   - host buffers are caller-owned; device memory is library-owned;
+ *   - a context is not thread-safe; distinct contexts are independent.
+ *
+ * Populations are stored structure-of-arrays [ip][iz][iy][ix] (x fastest),
+ * the reference layout (src/collision.cu:10, src/streaming.cu:9), with
+ * 64-bit offsets (the reference's int offsets overflow at 512^3).
+ */
+#ifndef LBM_DDF_H
+#define LBM_DDF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_E_INVALID (-1)   /* bad argument */
+#define LBM_E_HIP (-2)       /* HIP runtime error */
+#define LBM_E_NOMEM (-3)     /* device allocation failed */
+#define LBM_E_STATE (-4)     /* call not valid in the context's state */
+#define LBM_E_UNSUPPORTED (-5)
+#define LBM_E_IO (-6)
+
+/* Storage precision of the populations. */
+#define LBM_PREC_FP64 0 /* FP64 storage, FP64 arithmetic */
+#define LBM_PREC_FP16 1 /* FP16 deviations (scaled), FP32 arithmetic */
+#define LBM_PREC_BF16 2 /* BF16 deviations, FP32 arithmetic */
+
+/* Field selectors for lbm_get_field / lbm_set_field
+ * (the reference arrays d_rho, d_ux, d_uy, d_uz, d_phi, d_force_real*). */
+#define LBM_FIELD_RHO 0
+#define LBM_FIELD_UX 1
+#define LBM_FIELD_UY 2
+#define LBM_FIELD_UZ 3
+#define LBM_FIELD_PHI 4
+#define LBM_FIELD_FX 5
+#define LBM_FIELD_FY 6
+#define LBM_FIELD_FZ 7
+
+#define LBM_POP_F 0
+#define LBM_POP_G 1
+
+/* Physical and lattice parameters: the macros of src/parameters.h:8-26
+ * made runtime.  beta <= 0 selects the reference default 0.1/(5*LY). */
+typedef struct lbm_params {
+    int32_t lx, ly, lz;
+    int32_t precision; /* LBM_PREC_* */
+    double rayleigh;   /* RAYLEIGH */
+    double prandtl;    /* PRANDTL */
+    double t_hot;      /* T_HOT: wall at iy = 0 */
+    double t_cold;     /* T_COLD: wall at iy = LY-1 */
+    double gravity;    /* GRAVITY */
+    double beta;       /* BETA */
+} lbm_params;
+
+/* Derived parameters, src/globals.cu:11-23. */
+typedef struct lbm_derived {
+    double visc, tau, u0, diff, tauc, t0;
+} lbm_derived;
+
+/* Diagnostics of one completed step.
+ * nu_*: src/output.cu:199-256; *mean: src/output.cu:24-34;
+ * kinetic_energy = 0.5*<|u|^2>, mass = sum(rho'), max_speed = max|u| and
+ * nonfinite (count of non-finite moments) have no reference counterpart. */
+typedef struct lbm_diag {
+    int64_t step;
+    double nu_bulk, nu_top, nu_bottom;
+    double umean, vmean, wmean, tmean;
+    double kinetic_energy, mass;
+    double max_speed;
+    double nonfinite;
+} lbm_diag;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Message of the last failed call on this thread. */
+const char *lbm_last_error(void);
+
+/* src/globals.cu:11-23 (host-side parameter derivation). */
+int lbm_derive(const lbm_params *p, lbm_derived *out);
+
+/* Device bytes a context with these parameters allocates. */
+int lbm_bytes_required(const lbm_params *p, uint64_t *bytes);
+
+/* initialize(), src/initial.cu:3-67 (allocation) and :86-99 (constants).
+ * Allocates on the current HIP device; the state is undefined until an
+ * lbm_init_* call. */
+int lbm_create(const lbm_params *p, lbm_ctx **out);
+/* The frees of src/main.cu:98-117. */
+int lbm_destroy(lbm_ctx *ctx);
+
+/* init_arrays, src/initial.cu:208-242: rho'=0, u=0, phi=0.5, f=0, g=w*phi. */
+int lbm_init_arrays(lbm_ctx *ctx);
+/* Conductive start (no reference counterpart in src/; the profile is
+ * Parallel/initial.cu:89): T(iy) = T_hot - (T_hot-T_cold)*iy/(LY-1) plus
+ * amp*U(-1,1) from a counter-based generator keyed by (seed, cell index);
+ * rho'=0, u=0, f=0, g=w*phi. */
+int lbm_init_conductive(lbm_ctx *ctx, uint64_t seed, double amp);
+
+/* Stage interface (FP64 contexts only), one call per reference kernel
+ * launch of src/main.cu:27-45.  Each acts on the reference state.
+ *   lbm_forcing              forcing,              src/macrovar.cu:36-47
+ *   lbm_collision_BGK        collision_BGK,        src/collision.cu:3-50
+ *   lbm_collision_BGK_scalar collision_BGK_scalar, src/collision.cu:52-113
+ *   lbm_streaming            streaming + copy-back, src/streaming.cu:3-125, src/main.cu:39
+ *   lbm_streaming_scalar     streaming_scalar + copy-back, src/streaming.cu:127-251, src/main.cu:41
+ *   lbm_macrovar             macrovar,             src/macrovar.cu:3-34 */
+int lbm_forcing(lbm_ctx *ctx);
+int lbm_collision_BGK(lbm_ctx *ctx);
+int lbm_collision_BGK_scalar(lbm_ctx *ctx);
+int lbm_streaming(lbm_ctx *ctx);
+int lbm_streaming_scalar(lbm_ctx *ctx);
+int lbm_macrovar(lbm_ctx *ctx);
+
+/* nsteps iterations of the loop body of src/main.cu:27-48, each one fused
+ * collide-and-stream launch for f and g.  Asynchronous on the context's
+ * stream.  The last step's streaming half stays pending until a call that
+ * needs the reference state (lbm_finalize, lbm_get_*, lbm_reduce, the stage
+ * calls) completes it. */
+int lbm_step(lbm_ctx *ctx, int64_t nsteps);
+/* lbm_step, plus the elapsed device time of the nsteps fused launches
+ * (HIP events on the context's stream); waits for completion. */
+int lbm_step_timed(lbm_ctx *ctx, int64_t nsteps, double *elapsed_ms);
+/* lbm_step with the diagnostics of every completed step s with
+ * s % every == 0 folded into the fused launch (output_nu / diag_flow,
+ * src/main.cu:64-73).  Writes up to cap records; *nout receives the count.
+ * Ends with the reference state complete (as lbm_finalize). */
+int lbm_run_diag(lbm_ctx *ctx, int64_t nsteps, int64_t every, lbm_diag *out, int64_t cap,
+                 int64_t *nout);
+/* Complete a pending streaming half-step and the moments (macrovar). */
+int lbm_finalize(lbm_ctx *ctx);
+/* Diagnostics of the current state (output_nu, diag_flow; src/output.cu). */
+int lbm_reduce(lbm_ctx *ctx, lbm_diag *out);
+/* Per-y-plane <ux>,<uy>,<uz>,<phi>,phi_rms (output_profile,
+ * src/output.cu:282-377); out holds 5*LY doubles. */
+int lbm_profile(lbm_ctx *ctx, double *out);
+/* Number of completed reference steps. */
+int lbm_step_count(const lbm_ctx *ctx, int64_t *out);
+/* Wait for all work on the context's stream. */
+int lbm_synchronize(lbm_ctx *ctx);
+
+/* Host copies of the reference state (the D2H copies of src/output.cu).
+ * Fields: LX*LY*LZ doubles; populations: 27*LX*LY*LZ doubles. */
+int lbm_get_field(lbm_ctx *ctx, int32_t field, double *host);
+int lbm_set_field(lbm_ctx *ctx, int32_t field, const double *host);
+int lbm_get_pops(lbm_ctx *ctx, int32_t which, double *host);
+int lbm_set_pops(lbm_ctx *ctx, int32_t which, const double *host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,277 @@
+"""ctypes binding of the C ABI in include/lbm_ddf.h.
+
+The product path is the HIP library ``lbm_ddf_amd/_lib/liblbm_ddf.so``; this
+module only loads it and marshals host buffers.  There is no fallback: if the
+library is missing, loading raises.
+
+``Cavity`` mirrors the reference's stage interface (src/rb3d.h:47-68):
+``init_arrays`` (src/initial.cu:208-242), ``forcing`` (src/macrovar.cu:36-47),
+``collision_BGK`` / ``collision_BGK_scalar`` (src/collision.cu),
+``streaming`` / ``streaming_scalar`` (src/streaming.cu, with the copy-back of
+src/main.cu:39,41) and ``macrovar`` (src/macrovar.cu:3-34), plus the fused
+``step`` and the diagnostics of src/output.cu.  Errors raise ``LbmError``
+instead of exiting the process as the reference's CHECK_CUDA_ERROR does
+(src/rb3d.h:71-79).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "_lib" / "liblbm_ddf.so"
+
+PREC = {"fp64": 0, "fp16": 1, "bf16": 2}
+FIELDS = {"rho": 0, "ux": 1, "uy": 2, "uz": 3, "phi": 4, "fx": 5, "fy": 6, "fz": 7}
+POPS = {"f": 0, "g": 1}
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("lx", ctypes.c_int32), ("ly", ctypes.c_int32), ("lz", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("rayleigh", ctypes.c_double), ("prandtl", ctypes.c_double),
+        ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double),
+        ("gravity", ctypes.c_double), ("beta", ctypes.c_double),
+    ]
+
+
+class Derived(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("visc", "tau", "u0", "diff", "tauc", "t0")]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int64)] + [
+        (k, ctypes.c_double)
+        for k in ("nu_bulk", "nu_top", "nu_bottom", "umean", "vmean", "wmean", "tmean",
+                  "kinetic_energy", "mass", "max_speed", "nonfinite")
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+
+# Every entry point declared in include/lbm_ddf.h: name -> (restype, argtypes).
+SIGNATURES = {
+    "lbm_last_error": (ctypes.c_char_p, []),
+    "lbm_derive": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(Derived)]),
+    "lbm_bytes_required": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_uint64)]),
+    "lbm_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_P)]),
+    "lbm_destroy": (ctypes.c_int, [_P]),
+    "lbm_init_arrays": (ctypes.c_int, [_P]),
+    "lbm_init_conductive": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double]),
+    "lbm_forcing": (ctypes.c_int, [_P]),
+    "lbm_collision_BGK": (ctypes.c_int, [_P]),
+    "lbm_collision_BGK_scalar": (ctypes.c_int, [_P]),
+    "lbm_streaming": (ctypes.c_int, [_P]),
+    "lbm_streaming_scalar": (ctypes.c_int, [_P]),
+    "lbm_macrovar": (ctypes.c_int, [_P]),
+    "lbm_step": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "lbm_step_timed": (ctypes.c_int, [_P, ctypes.c_int64, _D]),
+    "lbm_run_diag": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Diag),
+                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "lbm_finalize": (ctypes.c_int, [_P]),
+    "lbm_reduce": (ctypes.c_int, [_P, ctypes.POINTER(Diag)]),
+    "lbm_profile": (ctypes.c_int, [_P, _D]),
+    "lbm_step_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "lbm_synchronize": (ctypes.c_int, [_P]),
+    "lbm_get_field": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+    "lbm_set_field": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+    "lbm_get_pops": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+    "lbm_set_pops": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+}
+
+
+def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
+    """Load the HIP library (in-tree build).  Raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise LbmError(f"HIP library not built: {p} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load().lbm_last_error()
+        raise LbmError(f"lbm error {rc}: {msg.decode() if msg else ''}")
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def make_params(lx, ly, lz, rayleigh=1.0e7, prandtl=0.71, t_hot=1.0, t_cold=0.0, gravity=1.0,
+                beta=0.0, precision="fp64") -> Params:
+    """Runtime form of the macros of src/parameters.h:8-26 (defaults are the
+    reference's; beta <= 0 selects 0.1/(5*LY))."""
+    return Params(int(lx), int(ly), int(lz), PREC[precision], float(rayleigh), float(prandtl),
+                  float(t_hot), float(t_cold), float(gravity), float(beta))
+
+
+def derive(params: Params) -> dict:
+    """src/globals.cu:11-23."""
+    d = Derived()
+    _check(load().lbm_derive(ctypes.byref(params), ctypes.byref(d)))
+    return {k: getattr(d, k) for k, _ in d._fields_}
+
+
+def bytes_required(params: Params) -> int:
+    v = ctypes.c_uint64()
+    _check(load().lbm_bytes_required(ctypes.byref(params), ctypes.byref(v)))
+    return int(v.value)
+
+
+class Cavity:
+    """One Rayleigh-Benard cavity on the current HIP device."""
+
+    def __init__(self, lx, ly, lz, **kw):
+        self.lib = load()
+        self.params = make_params(lx, ly, lz, **kw)
+        self.shape = (int(lz), int(ly), int(lx))
+        self.n = int(lx) * int(ly) * int(lz)
+        h = _P()
+        _check(self.lib.lbm_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- initial state ----------------------------------------------------
+    def init_arrays(self):
+        _check(self.lib.lbm_init_arrays(self.h))
+
+    def init_conductive(self, seed: int, amp: float = 1.0e-3):
+        _check(self.lib.lbm_init_conductive(self.h, ctypes.c_uint64(seed), float(amp)))
+
+    # -- stage interface --------------------------------------------------
+    def forcing(self):
+        _check(self.lib.lbm_forcing(self.h))
+
+    def collision_BGK(self):
+        _check(self.lib.lbm_collision_BGK(self.h))
+
+    def collision_BGK_scalar(self):
+        _check(self.lib.lbm_collision_BGK_scalar(self.h))
+
+    def streaming(self):
+        _check(self.lib.lbm_streaming(self.h))
+
+    def streaming_scalar(self):
+        _check(self.lib.lbm_streaming_scalar(self.h))
+
+    def macrovar(self):
+        _check(self.lib.lbm_macrovar(self.h))
+
+    def staged_step(self, n: int = 1):
+        """n iterations of src/main.cu:27-48 through the stage entry points."""
+        for _ in range(n):
+            self.forcing()
+            self.collision_BGK()
+            self.collision_BGK_scalar()
+            self.streaming()
+            self.streaming_scalar()
+            self.macrovar()
+
+    # -- fused path -------------------------------------------------------
+    def step(self, n: int = 1):
+        _check(self.lib.lbm_step(self.h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib.lbm_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def run_diag(self, n: int, every: int) -> list:
+        cap = n // every + 1
+        arr = (Diag * cap)()
+        nout = ctypes.c_int64()
+        _check(self.lib.lbm_run_diag(self.h, int(n), int(every), arr, cap, ctypes.byref(nout)))
+        return [arr[i].as_dict() for i in range(nout.value)]
+
+    def finalize(self):
+        _check(self.lib.lbm_finalize(self.h))
+
+    def synchronize(self):
+        _check(self.lib.lbm_synchronize(self.h))
+
+    @property
+    def step_count(self) -> int:
+        v = ctypes.c_int64()
+        _check(self.lib.lbm_step_count(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    # -- diagnostics ------------------------------------------------------
+    def reduce(self) -> dict:
+        d = Diag()
+        _check(self.lib.lbm_reduce(self.h, ctypes.byref(d)))
+        return d.as_dict()
+
+    def profile(self) -> np.ndarray:
+        out = np.zeros((self.shape[1], 5), dtype=np.float64)
+        _check(self.lib.lbm_profile(self.h, _dptr(out)))
+        return out
+
+    # -- host copies ------------------------------------------------------
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(self.lib.lbm_get_field(self.h, FIELDS[name], _dptr(out)))
+        return out
+
+    def set_field(self, name: str, values) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), self.shape))
+        _check(self.lib.lbm_set_field(self.h, FIELDS[name], _dptr(a)))
+
+    def get_pops(self, which: str) -> np.ndarray:
+        out = np.empty((27,) + self.shape, dtype=np.float64)
+        _check(self.lib.lbm_get_pops(self.h, POPS[which], _dptr(out)))
+        return out
+
+    def set_pops(self, which: str, values) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (27,) + self.shape))
+        _check(self.lib.lbm_set_pops(self.h, POPS[which], _dptr(a)))
+
+    def state(self) -> dict:
+        """The reference state: f, g, rho', u, phi."""
+        return {
+            "f": self.get_pops("f"), "g": self.get_pops("g"),
+            "rho": self.get_field("rho"), "ux": self.get_field("ux"), "uy": self.get_field("uy"),
+            "uz": self.get_field("uz"), "phi": self.get_field("phi"),
+        }

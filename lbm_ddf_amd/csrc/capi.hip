@@ -1,0 +1,580 @@
+// capi.hip -- the C ABI of include/lbm_ddf.h: context, phases, host copies.
+//
+// A context holds two population buffers per distribution (pull A/B scheme),
+// the reference-state moment fields, and the two per-cell side values of the
+// fused kernel.  It is in one of three phases:
+//   EMPTY  allocated, no state yet;
+//   REF    populations are the reference's post-streaming f, g and the
+//          fields rho', u, phi, F_y are the reference's (as after
+//          src/main.cu:45);
+//   FUSED  populations are post-collision (the streaming half of the last
+//          step is pending), with lag phi and wall rho' valid.
+// Calls that need the reference state run the finalize pass first.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(LBM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define LBM_HIP(call)                                      \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+    } while (0)
+
+enum Phase { PHASE_EMPTY = 0, PHASE_REF = 1, PHASE_FUSED = 2 };
+
+size_t elem_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 2; }
+size_t side_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 4; }
+
+int default_block(int lx) {
+    if (lx <= 64) return 64;
+    if (lx <= 128) return 128;
+    return 256;
+}
+
+int validate(const lbm_params *p) {
+    if (!p) return fail(LBM_E_INVALID, "params is NULL");
+    if (p->lx < 1 || p->ly < 3 || p->lz < 1) return fail(LBM_E_INVALID, "need lx >= 1, ly >= 3, lz >= 1");
+    const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
+    if (n >= (1ull << 32)) return fail(LBM_E_INVALID, "cell count must be below 2^32");
+    if (p->precision < LBM_PREC_FP64 || p->precision > LBM_PREC_BF16)
+        return fail(LBM_E_INVALID, "unknown precision");
+    if (!(p->rayleigh > 0.0) || !(p->prandtl > 0.0)) return fail(LBM_E_INVALID, "Ra and Pr must be > 0");
+    return LBM_OK;
+}
+
+}  // namespace
+
+struct lbm_ctx {
+    lbm_params p{};
+    lbm_derived d{};
+    double beta = 0.0;
+    uint64_t n = 0;
+    int prec = 0;
+    void *f[2] = {nullptr, nullptr};
+    void *g[2] = {nullptr, nullptr};
+    int cf = 0, cg = 0;
+    double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *phi = nullptr, *fy = nullptr;
+    void *lag = nullptr;
+    void *wall_rho = nullptr;
+    double *partials = nullptr;
+    lbm_diag *diag_dev = nullptr;
+    int64_t diag_cap = 0;
+    double *plane = nullptr;  // one population plane in FP64 (codec staging)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int phase = PHASE_EMPTY;
+    int64_t nstep = 0;
+    int bx = 256;
+    int device = 0;
+};
+
+namespace {
+
+lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, void *gb) {
+    lbm::StepArgs a{};
+    a.fa = fa;
+    a.ga = ga;
+    a.fb = fb;
+    a.gb = gb;
+    a.n = c->n;
+    a.lx = c->p.lx;
+    a.ly = c->p.ly;
+    a.lz = c->p.lz;
+    a.ntx = (c->p.lx + c->bx - 1) / c->bx;
+    a.tau = c->d.tau;
+    a.tauc = c->d.tauc;
+    a.c1 = 1.0 - 0.5 / c->d.tau;
+    a.grav_beta = c->p.gravity * c->beta;
+    a.t_hot = c->p.t_hot;
+    a.t_cold = c->p.t_cold;
+    a.t0 = c->d.t0;
+    a.lag = c->lag;
+    a.wall_rho = c->wall_rho;
+    a.rho = c->rho;
+    a.ux = c->ux;
+    a.uy = c->uy;
+    a.uz = c->uz;
+    a.phi = c->phi;
+    a.fy = c->fy;
+    a.partials = c->partials;
+    return a;
+}
+
+hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
+    switch (c->prec) {
+        case LBM_PREC_FP64: return lbm::launch_step_f64(a, mode, reduce, c->bx, c->stream);
+        case LBM_PREC_FP16: return lbm::launch_step_f16(a, mode, reduce, c->bx, c->stream);
+        case LBM_PREC_BF16: return lbm::launch_step_bf16(a, mode, reduce, c->bx, c->stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// One pass of the given mode from the current buffers into the others.
+int run_pass(lbm_ctx *c, int mode, bool reduce) {
+    lbm::StepArgs a = make_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
+    LBM_HIP(launch_step(c, a, mode, reduce));
+    c->cf = 1 - c->cf;
+    c->cg = 1 - c->cg;
+    return LBM_OK;
+}
+
+lbm::DiagArgs diag_args(const lbm_ctx *c, int64_t step) {
+    lbm::DiagArgs d{};
+    d.lx = c->p.lx;
+    d.ly = c->p.ly;
+    d.lz = c->p.lz;
+    d.diff = c->d.diff;
+    d.dT = c->p.t_hot - c->p.t_cold;
+    d.step = step;
+    return d;
+}
+
+int ensure_diag_cap(lbm_ctx *c, int64_t cap) {
+    if (cap <= c->diag_cap) return LBM_OK;
+    if (c->diag_dev) {
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        LBM_HIP(hipFree(c->diag_dev));
+        c->diag_dev = nullptr;
+        c->diag_cap = 0;
+    }
+    if (hipMalloc(&c->diag_dev, sizeof(lbm_diag) * cap) != hipSuccess) return fail(LBM_E_NOMEM, "diag buffer");
+    c->diag_cap = cap;
+    return LBM_OK;
+}
+
+int64_t nblocks(const lbm_ctx *c) { return lbm::step_grid_blocks(c->p.lx, c->p.ly, c->p.lz, c->bx); }
+
+// Completes a pending streaming half-step (FUSED -> REF), optionally with the
+// diagnostics of that step written to diag_dev[slot].
+int finalize(lbm_ctx *c, bool reduce, int64_t slot) {
+    if (c->phase != PHASE_FUSED) return LBM_OK;
+    int rc = run_pass(c, lbm::MODE_FINALIZE, reduce);
+    if (rc) return rc;
+    if (reduce) LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev + slot, c->stream));
+    c->phase = PHASE_REF;
+    return LBM_OK;
+}
+
+int ensure_ref(lbm_ctx *c) {
+    if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
+    return finalize(c, false, 0);
+}
+
+int check_ctx(const lbm_ctx *c) {
+    if (!c) return fail(LBM_E_INVALID, "ctx is NULL");
+    return LBM_OK;
+}
+
+int require_fp64(const lbm_ctx *c, const char *what) {
+    if (c->prec != LBM_PREC_FP64)
+        return fail(LBM_E_UNSUPPORTED, std::string(what) + " is available for FP64 contexts only");
+    return LBM_OK;
+}
+
+double *field_ptr(lbm_ctx *c, int field) {
+    switch (field) {
+        case LBM_FIELD_RHO: return c->rho;
+        case LBM_FIELD_UX: return c->ux;
+        case LBM_FIELD_UY: return c->uy;
+        case LBM_FIELD_UZ: return c->uz;
+        case LBM_FIELD_PHI: return c->phi;
+        case LBM_FIELD_FY: return c->fy;
+        default: return nullptr;
+    }
+}
+
+void derive(const lbm_params *p, lbm_derived *d, double *beta_out) {
+    // src/globals.cu:11-23; BETA default src/parameters.h:26.
+    const double beta = p->beta > 0.0 ? p->beta : 0.1 / ((double)p->ly * 5.0);
+    const double grav0 = p->gravity, tHot = p->t_hot, tCold = p->t_cold;
+    d->visc = (double)p->ly * std::sqrt(grav0 * beta * (tHot - tCold) * (double)p->ly * p->prandtl / p->rayleigh);
+    d->tau = 3.0 * d->visc + 0.5;
+    d->u0 = std::sqrt(grav0 * beta * (tHot - tCold) * (double)p->ly);
+    d->diff = d->visc / p->prandtl;
+    d->tauc = 3.0 * d->diff + 0.5;
+    d->t0 = 0.5 * (tCold + tHot);
+    if (beta_out) *beta_out = beta;
+}
+
+uint64_t bytes_for(const lbm_params *p, int bx) {
+    const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
+    const uint64_t pops = 4ull * 27ull * n * elem_size(p->precision);
+    const uint64_t fields = 6ull * n * 8ull;
+    const uint64_t side = n * side_size(p->precision) + 2ull * p->lx * p->lz * side_size(p->precision);
+    const uint64_t part = static_cast<uint64_t>(lbm::step_grid_blocks(p->lx, p->ly, p->lz, bx)) * lbm::NQ * 8ull;
+    return pops + fields + side + part;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_last_error(void) { return g_last_error.c_str(); }
+
+int lbm_derive(const lbm_params *p, lbm_derived *out) {
+    if (!p || !out) return fail(LBM_E_INVALID, "NULL argument");
+    if (p->ly < 1) return fail(LBM_E_INVALID, "ly must be >= 1");
+    derive(p, out, nullptr);
+    return LBM_OK;
+}
+
+int lbm_bytes_required(const lbm_params *p, uint64_t *bytes) {
+    int rc = validate(p);
+    if (rc) return rc;
+    if (!bytes) return fail(LBM_E_INVALID, "bytes is NULL");
+    *bytes = bytes_for(p, default_block(p->lx));
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx *c) {
+    if (!c) return LBM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->f[0], c->f[1], c->g[0], c->g[1], c->rho, c->ux, c->uy, c->uz, c->phi, c->fy,
+                    c->lag, c->wall_rho, c->partials, c->diag_dev, c->plane};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_create(const lbm_params *p, lbm_ctx **out) {
+    int rc = validate(p);
+    if (rc) return rc;
+    if (!out) return fail(LBM_E_INVALID, "out is NULL");
+    *out = nullptr;
+    lbm_ctx *c = new lbm_ctx();
+    c->p = *p;
+    c->prec = p->precision;
+    c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
+    derive(p, &c->d, &c->beta);
+    c->bx = default_block(p->lx);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) {
+        lbm_destroy(c);
+        return hip_fail(e, "hipGetDevice");
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        lbm_destroy(c);
+        return hip_fail(e, "stream/event creation");
+    }
+    const uint64_t n = c->n;
+    const size_t pop_bytes = 27ull * n * elem_size(c->prec);
+    struct Alloc {
+        void **ptr;
+        size_t bytes;
+    } allocs[] = {
+        {&c->f[0], pop_bytes},
+        {&c->f[1], pop_bytes},
+        {&c->g[0], pop_bytes},
+        {&c->g[1], pop_bytes},
+        {reinterpret_cast<void **>(&c->rho), n * 8},
+        {reinterpret_cast<void **>(&c->ux), n * 8},
+        {reinterpret_cast<void **>(&c->uy), n * 8},
+        {reinterpret_cast<void **>(&c->uz), n * 8},
+        {reinterpret_cast<void **>(&c->phi), n * 8},
+        {reinterpret_cast<void **>(&c->fy), n * 8},
+        {&c->lag, n * side_size(c->prec)},
+        {&c->wall_rho, 2ull * p->lx * p->lz * side_size(c->prec)},
+        {reinterpret_cast<void **>(&c->partials), static_cast<size_t>(nblocks(c)) * lbm::NQ * 8},
+    };
+    for (auto &al : allocs) {
+        if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            lbm_destroy(c);
+            return fail(LBM_E_NOMEM, "device allocation of " + std::to_string(al.bytes) + " bytes failed");
+        }
+    }
+    rc = ensure_diag_cap(c, 16);
+    if (rc) {
+        lbm_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_init_arrays(lbm_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
+    LBM_HIP(lbm::launch_init(a, c->prec, 0, 0, 0.0, c->stream));
+    c->phase = PHASE_REF;
+    c->nstep = 0;
+    return LBM_OK;
+}
+
+int lbm_init_conductive(lbm_ctx *c, uint64_t seed, double amp) {
+    if (int rc = check_ctx(c)) return rc;
+    lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
+    LBM_HIP(lbm::launch_init(a, c->prec, 1, seed, amp, c->stream));
+    c->phase = PHASE_REF;
+    c->nstep = 0;
+    return LBM_OK;
+}
+
+// ---- stage interface ------------------------------------------------------
+
+#define LBM_STAGE_PROLOGUE(name)                      \
+    if (int rc = check_ctx(c)) return rc;             \
+    if (int rc = require_fp64(c, name)) return rc;    \
+    if (int rc = ensure_ref(c)) return rc
+
+int lbm_forcing(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_forcing");
+    LBM_HIP(lbm::launch_forcing(make_args(c, nullptr, nullptr, nullptr, nullptr), c->stream));
+    return LBM_OK;
+}
+
+int lbm_collision_BGK(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_collision_BGK");
+    LBM_HIP(lbm::launch_collide_f(make_args(c, nullptr, nullptr, c->f[c->cf], nullptr), c->stream));
+    return LBM_OK;
+}
+
+int lbm_collision_BGK_scalar(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_collision_BGK_scalar");
+    LBM_HIP(lbm::launch_collide_g(make_args(c, nullptr, nullptr, nullptr, c->g[c->cg]), c->stream));
+    return LBM_OK;
+}
+
+int lbm_streaming(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_streaming");
+    LBM_HIP(lbm::launch_stream_f(make_args(c, c->f[c->cf], nullptr, c->f[1 - c->cf], nullptr), c->stream));
+    c->cf = 1 - c->cf;
+    return LBM_OK;
+}
+
+int lbm_streaming_scalar(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_streaming_scalar");
+    LBM_HIP(lbm::launch_stream_g(make_args(c, nullptr, c->g[c->cg], nullptr, c->g[1 - c->cg]), c->stream));
+    c->cg = 1 - c->cg;
+    return LBM_OK;
+}
+
+int lbm_macrovar(lbm_ctx *c) {
+    LBM_STAGE_PROLOGUE("lbm_macrovar");
+    LBM_HIP(lbm::launch_macrovar(make_args(c, c->f[c->cf], c->g[c->cg], nullptr, nullptr), c->stream));
+    c->nstep += 1;
+    return LBM_OK;
+}
+
+// ---- fused path -----------------------------------------------------------
+
+int lbm_step(lbm_ctx *c, int64_t nsteps) {
+    if (int rc = check_ctx(c)) return rc;
+    if (nsteps < 0) return fail(LBM_E_INVALID, "nsteps < 0");
+    if (nsteps == 0) return LBM_OK;
+    if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
+    int64_t fused = nsteps;
+    if (c->phase == PHASE_REF) {
+        if (int rc = run_pass(c, lbm::MODE_PROLOGUE, false)) return rc;
+        c->phase = PHASE_FUSED;
+        fused -= 1;
+    }
+    for (int64_t k = 0; k < fused; ++k)
+        if (int rc = run_pass(c, lbm::MODE_FUSED, false)) return rc;
+    c->nstep += nsteps;
+    return LBM_OK;
+}
+
+int lbm_step_timed(lbm_ctx *c, int64_t nsteps, double *elapsed_ms) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!elapsed_ms) return fail(LBM_E_INVALID, "elapsed_ms is NULL");
+    LBM_HIP(hipEventRecord(c->ev0, c->stream));
+    if (int rc = lbm_step(c, nsteps)) return rc;
+    LBM_HIP(hipEventRecord(c->ev1, c->stream));
+    LBM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    LBM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *elapsed_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_run_diag(lbm_ctx *c, int64_t nsteps, int64_t every, lbm_diag *out, int64_t cap, int64_t *nout) {
+    if (int rc = check_ctx(c)) return rc;
+    if (nsteps < 0 || every <= 0 || cap < 0 || (cap > 0 && !out))
+        return fail(LBM_E_INVALID, "bad nsteps/every/cap/out");
+    if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
+    const int64_t start = c->nstep;  // diagnostics for completed steps s in (start, start+nsteps]
+    const int64_t end = start + nsteps;
+    if (int rc = ensure_diag_cap(c, cap > 0 ? cap : 1)) return rc;
+    int64_t nrec = 0;
+    if (nsteps > 0) {
+        int64_t todo = nsteps;
+        // In FUSED phase the step `start` is collided and its streaming half
+        // is pending; from REF the prologue collides step start+1.
+        int64_t pending = start;
+        if (c->phase == PHASE_REF) {
+            if (int rc = run_pass(c, lbm::MODE_PROLOGUE, false)) return rc;
+            c->phase = PHASE_FUSED;
+            todo -= 1;
+            pending = start + 1;
+        }
+        for (int64_t k = 0; k < todo; ++k) {
+            const int64_t s = pending + k;
+            const bool red = s > start && s % every == 0 && nrec < cap;
+            if (int rc = run_pass(c, lbm::MODE_FUSED, red)) return rc;
+            if (red) {
+                LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, s), c->diag_dev + nrec, c->stream));
+                ++nrec;
+            }
+        }
+        c->nstep = end;
+        const bool red = end % every == 0 && nrec < cap;
+        if (int rc = finalize(c, red, nrec)) return rc;
+        if (red) ++nrec;
+    } else {
+        if (int rc = ensure_ref(c)) return rc;
+    }
+    if (nrec > 0) LBM_HIP(hipMemcpyAsync(out, c->diag_dev, sizeof(lbm_diag) * nrec, hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    if (nout) *nout = nrec;
+    return LBM_OK;
+}
+
+int lbm_finalize(lbm_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    return ensure_ref(c);
+}
+
+int lbm_reduce(lbm_ctx *c, lbm_diag *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!out) return fail(LBM_E_INVALID, "out is NULL");
+    if (int rc = ensure_ref(c)) return rc;
+    LBM_HIP(lbm::launch_reduce_fields(make_args(c, nullptr, nullptr, nullptr, nullptr), c->bx, c->stream));
+    LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev, c->stream));
+    LBM_HIP(hipMemcpyAsync(out, c->diag_dev, sizeof(lbm_diag), hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_profile(lbm_ctx *c, double *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!out) return fail(LBM_E_INVALID, "out is NULL");
+    if (int rc = ensure_ref(c)) return rc;
+    double *dev = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&dev), sizeof(double) * 5 * c->p.ly) != hipSuccess)
+        return fail(LBM_E_NOMEM, "profile buffer");
+    hipError_t e = lbm::launch_profile(make_args(c, nullptr, nullptr, nullptr, nullptr), dev, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, sizeof(double) * 5 * c->p.ly, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_fail(e, "profile");
+    return LBM_OK;
+}
+
+int lbm_step_count(const lbm_ctx *c, int64_t *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!out) return fail(LBM_E_INVALID, "out is NULL");
+    *out = c->nstep;
+    return LBM_OK;
+}
+
+int lbm_synchronize(lbm_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+// ---- host copies ----------------------------------------------------------
+
+int lbm_get_field(lbm_ctx *c, int32_t field, double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!host) return fail(LBM_E_INVALID, "host is NULL");
+    if (field < LBM_FIELD_RHO || field > LBM_FIELD_FZ) return fail(LBM_E_INVALID, "unknown field");
+    if (int rc = ensure_ref(c)) return rc;
+    if (field == LBM_FIELD_FX || field == LBM_FIELD_FZ) {
+        // The reference writes 0.0 to F_x and F_z (src/macrovar.cu:44,46).
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        std::memset(host, 0, sizeof(double) * c->n);
+        return LBM_OK;
+    }
+    LBM_HIP(hipMemcpyAsync(host, field_ptr(c, field), sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_set_field(lbm_ctx *c, int32_t field, const double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!host) return fail(LBM_E_INVALID, "host is NULL");
+    if (field < LBM_FIELD_RHO || field > LBM_FIELD_FZ) return fail(LBM_E_INVALID, "unknown field");
+    if (int rc = ensure_ref(c)) return rc;
+    if (field == LBM_FIELD_FX || field == LBM_FIELD_FZ) {
+        for (uint64_t i = 0; i < c->n; ++i)
+            if (host[i] != 0.0) return fail(LBM_E_UNSUPPORTED, "F_x and F_z are identically zero in this model");
+        return LBM_OK;
+    }
+    LBM_HIP(hipMemcpyAsync(field_ptr(c, field), host, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_pops(lbm_ctx *c, int32_t which, double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!host || (which != LBM_POP_F && which != LBM_POP_G)) return fail(LBM_E_INVALID, "bad which/host");
+    if (int rc = ensure_ref(c)) return rc;
+    void *src = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
+    if (c->prec == LBM_PREC_FP64) {
+        LBM_HIP(hipMemcpyAsync(host, src, sizeof(double) * 27 * c->n, hipMemcpyDeviceToHost, c->stream));
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        return LBM_OK;
+    }
+    if (!c->plane && hipMalloc(&c->plane, sizeof(double) * c->n) != hipSuccess)
+        return fail(LBM_E_NOMEM, "plane staging buffer");
+    const size_t es = elem_size(c->prec);
+    for (int ip = 0; ip < 27; ++ip) {
+        const void *ps = static_cast<const char *>(src) + static_cast<size_t>(ip) * c->n * es;
+        LBM_HIP(lbm::launch_decode_plane(c->prec, which, ip, ps, c->plane, c->n, c->d.t0, c->stream));
+        LBM_HIP(hipMemcpyAsync(host + static_cast<size_t>(ip) * c->n, c->plane, sizeof(double) * c->n,
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_set_pops(lbm_ctx *c, int32_t which, const double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!host || (which != LBM_POP_F && which != LBM_POP_G)) return fail(LBM_E_INVALID, "bad which/host");
+    if (int rc = ensure_ref(c)) return rc;
+    void *dst = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
+    if (c->prec == LBM_PREC_FP64) {
+        LBM_HIP(hipMemcpyAsync(dst, host, sizeof(double) * 27 * c->n, hipMemcpyHostToDevice, c->stream));
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        return LBM_OK;
+    }
+    if (!c->plane && hipMalloc(&c->plane, sizeof(double) * c->n) != hipSuccess)
+        return fail(LBM_E_NOMEM, "plane staging buffer");
+    const size_t es = elem_size(c->prec);
+    for (int ip = 0; ip < 27; ++ip) {
+        void *pd = static_cast<char *>(dst) + static_cast<size_t>(ip) * c->n * es;
+        LBM_HIP(hipMemcpyAsync(c->plane, host + static_cast<size_t>(ip) * c->n, sizeof(double) * c->n,
+                               hipMemcpyHostToDevice, c->stream));
+        LBM_HIP(lbm::launch_encode_plane(c->prec, which, ip, c->plane, pd, c->n, c->d.t0, c->stream));
+    }
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+}  // extern "C"

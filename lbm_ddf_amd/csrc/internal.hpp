@@ -1,0 +1,92 @@
+// internal.hpp -- shared declarations of the HIP library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "lbm_ddf.h"
+
+namespace lbm {
+
+// Kernel modes of the one-pass step kernel.
+enum StepMode : int {
+    MODE_PROLOGUE = 0,  // forcing + collide f, g of the reference state (no streaming)
+    MODE_FUSED = 1,     // stream + moments of step k, forcing + collide of step k+1
+    MODE_FINALIZE = 2,  // stream + moments of step k; writes the reference state
+};
+
+// Number of per-block partial sums of the folded diagnostics.
+constexpr int NQ = 16;
+enum DiagQ : int {
+    Q_UYPHI = 0,  // sum u_y*phi                     (src/output.cu:199-201,228-231)
+    Q_UX, Q_UY, Q_UZ, Q_PHI,  // sums for the means  (src/output.cu:24-30)
+    Q_USQ,        // sum |u|^2
+    Q_RHO,        // sum rho'
+    Q_B0, Q_B1, Q_B2,  // sum phi over the planes iy = 0, 1, 2
+    Q_T0, Q_T1, Q_T2,  // sum phi over the planes iy = LY-1, LY-2, LY-3
+    Q_MAXUSQ,     // max |u|^2
+    Q_NONFINITE,  // count of cells with a non-finite moment
+    Q_PAD
+};
+
+// Arguments of the step kernels (one cavity).  Pointers are device pointers.
+struct StepArgs {
+    const void *fa, *ga;  // source populations [27][n]
+    void *fb, *gb;        // destination populations [27][n]
+    uint64_t n;           // cells per population plane
+    int32_t lx, ly, lz;
+    int32_t ntx;          // blocks per x-row
+    double tau, tauc;     // relaxation times (src/globals.cu:14,17)
+    double c1;            // 1 - 0.5/tau (src/collision.cu:31)
+    double grav_beta;     // grav0*beta (src/macrovar.cu:45)
+    double t_hot, t_cold, t0;
+    void *lag;            // phi of the previous completed step [n]
+    void *wall_rho;       // rho' of the previous completed step at iy=0 and iy=LY-1 [2][lz][lx]
+    // reference-state fields (prologue reads, finalize writes)
+    double *rho, *ux, *uy, *uz, *phi, *fy;
+    double *partials;     // [gridDim.x][NQ] when reducing
+};
+
+// Geometry/physics of one cavity for the diagnostics finish kernel.
+struct DiagArgs {
+    int32_t lx, ly, lz;
+    double diff, dT;
+    int64_t step;
+};
+
+// Launchers (defined in step_f64.hip / step_lp.hip / stage_f64.hip / reduce.hip).
+hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
+hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
+hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
+
+// Reference-state stage kernels (FP64).
+hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
+                       hipStream_t s);
+hipError_t launch_forcing(const StepArgs &a, hipStream_t s);
+hipError_t launch_collide_f(const StepArgs &a, hipStream_t s);
+hipError_t launch_collide_g(const StepArgs &a, hipStream_t s);
+hipError_t launch_stream_f(const StepArgs &a, hipStream_t s);
+hipError_t launch_stream_g(const StepArgs &a, hipStream_t s);
+hipError_t launch_macrovar(const StepArgs &a, hipStream_t s);
+
+// Diagnostics over the reference-state fields; the finish kernel turns the
+// per-block partials into an lbm_diag at dst.
+hipError_t launch_reduce_fields(const StepArgs &a, int bx, hipStream_t s);
+hipError_t launch_diag_finish(const double *partials, int64_t nblocks, const DiagArgs &d,
+                              lbm_diag *dst, hipStream_t s);
+hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s);
+
+// Population plane conversion between the storage format and FP64.
+hipError_t launch_decode_plane(int precision, int which, int ip, const void *src, double *dst,
+                               uint64_t n, double t0, hipStream_t s);
+hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst,
+                               uint64_t n, double t0, hipStream_t s);
+
+inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
+    int64_t ntx = (lx + bx - 1) / bx;
+    return ntx * (int64_t)ly * (int64_t)lz;
+}
+
+}  // namespace lbm

@@ -1,0 +1,418 @@
+// step_kernel.hpp -- the one-pass D3Q27 double-distribution step kernel.
+//
+// One thread owns one lattice cell.  A fused launch does, for every cell x,
+// what the reference does in five kernels and two device copies
+// (src/main.cu:27-48):
+//
+//   gather  f_i(x) = f*_i(x - c_i), or the wall rule        (src/streaming.cu:3-125)
+//           g_i(x) = g*_i(x - c_i), or the wall rule        (src/streaming.cu:127-251)
+//   moments rho', j, phi; u = j + F(phi of step k-1)/2       (src/macrovar.cu:3-34)
+//   forcing F = g*beta*(phi - t0)                            (src/macrovar.cu:36-47)
+//   collide f (BGK + buoyancy), g (BGK)                      (src/collision.cu:3-81)
+//   store   f*, g* to the other buffer (pull A/B scheme)
+//
+// plus, optionally, the per-cell terms of the diagnostics (src/output.cu).
+//
+// Two per-cell side values carry what the reference keeps in full fields:
+//   lag[x]      phi of the previous completed step, because the reference
+//               adds half of the force computed from the previous phi to u
+//               (src/macrovar.cu:31 with src/main.cu:27);
+//   wall_rho[x] rho' of the previous completed step on the planes iy = 0 and
+//               iy = LY-1, read by the isothermal anti-bounce-back
+//               (src/streaming.cu:206-227).
+// Each is read and rewritten only by the thread that owns the cell, so the
+// update is in place.
+//
+// FP64 storage keeps the reference's operation order (divisions by RT, tau
+// and tauc included) and is compiled with -ffp-contract=off; the result is
+// then the same as the FP64 oracle bit for bit.  16-bit storage keeps
+// deviations (f as stored by the reference, g - w_i*t0) and computes in FP32.
+#pragma once
+
+#include "internal.hpp"
+#include "lattice.hpp"
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Storage policies
+// ---------------------------------------------------------------------------
+
+struct PolicyF64 {
+    using S = double;  // stored population
+    using T = double;  // arithmetic
+    using L = double;  // side values (lag phi, wall rho')
+    static constexpr bool exact = true;
+    __device__ __forceinline__ static T dec_f(S s) { return s; }
+    __device__ __forceinline__ static S enc_f(T t) { return t; }
+    template <int I>
+    __device__ __forceinline__ static T dec_g(S s, T) { return s; }
+    template <int I>
+    __device__ __forceinline__ static S enc_g(T t, T) { return t; }
+};
+
+// FP16: value * 2^15 stored as IEEE binary16 (round to nearest even).  The
+// power-of-two scale keeps the small deviations out of the subnormal range.
+struct PolicyF16 {
+    using S = _Float16;
+    using T = float;
+    using L = float;
+    static constexpr bool exact = false;
+    static constexpr float kScale = 32768.0f;
+    static constexpr float kInv = 1.0f / 32768.0f;
+    __device__ __forceinline__ static T dec_f(S s) { return static_cast<float>(s) * kInv; }
+    __device__ __forceinline__ static S enc_f(T t) { return static_cast<_Float16>(t * kScale); }
+    template <int I>
+    __device__ __forceinline__ static T dec_g(S s, T t0) {
+        return static_cast<float>(s) * kInv + static_cast<float>(weight(I)) * t0;
+    }
+    template <int I>
+    __device__ __forceinline__ static S enc_g(T t, T t0) {
+        return static_cast<_Float16>((t - static_cast<float>(weight(I)) * t0) * kScale);
+    }
+};
+
+// BF16: the upper 16 bits of the binary32 value, rounded to nearest even.
+__device__ __forceinline__ float bf16_bits_to_float(uint16_t b) {
+    return __uint_as_float(static_cast<uint32_t>(b) << 16);
+}
+__device__ __forceinline__ uint16_t float_to_bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu) != 0u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+struct PolicyBF16 {
+    using S = uint16_t;
+    using T = float;
+    using L = float;
+    static constexpr bool exact = false;
+    __device__ __forceinline__ static T dec_f(S s) { return bf16_bits_to_float(s); }
+    __device__ __forceinline__ static S enc_f(T t) { return float_to_bf16_bits(t); }
+    template <int I>
+    __device__ __forceinline__ static T dec_g(S s, T t0) {
+        return bf16_bits_to_float(s) + static_cast<float>(weight(I)) * t0;
+    }
+    template <int I>
+    __device__ __forceinline__ static S enc_g(T t, T t0) {
+        return float_to_bf16_bits(t - static_cast<float>(weight(I)) * t0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Block reduction of the diagnostics (deterministic: fixed shuffle tree, then
+// waves summed in order by one thread per quantity).
+// ---------------------------------------------------------------------------
+
+template <int BX>
+__device__ __forceinline__ void block_reduce_diag(double (&q)[NQ], double *partials) {
+    static_assert(BX % 64 == 0, "block must be whole waves");
+    constexpr int NW = BX / 64;
+    __shared__ double red[NW][NQ];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        double v = q[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            double o = __shfl_xor(v, off, 64);
+            v = (k == Q_MAXUSQ) ? fmax(v, o) : v + o;
+        }
+        q[k] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) red[wave][k] = q[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int k = threadIdx.x;
+        double v = red[0][k];
+        for (int w = 1; w < NW; ++w) v = (k == Q_MAXUSQ) ? fmax(v, red[w][k]) : v + red[w][k];
+        partials[static_cast<uint64_t>(blockIdx.x) * NQ + k] = v;
+    }
+}
+
+// Per-cell diagnostic terms of a completed step.
+__device__ __forceinline__ void diag_terms(double (&q)[NQ], bool active, int iy, int ly, double rho,
+                                           double ux, double uy, double uz, double phi) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    if (!active) return;
+    const double usq = ux * ux + uy * uy + uz * uz;
+    q[Q_UYPHI] = uy * phi;
+    q[Q_UX] = ux;
+    q[Q_UY] = uy;
+    q[Q_UZ] = uz;
+    q[Q_PHI] = phi;
+    q[Q_USQ] = usq;
+    q[Q_RHO] = rho;
+    q[Q_B0] = iy == 0 ? phi : 0.0;
+    q[Q_B1] = iy == 1 ? phi : 0.0;
+    q[Q_B2] = iy == 2 ? phi : 0.0;
+    q[Q_T0] = iy == ly - 1 ? phi : 0.0;
+    q[Q_T1] = iy == ly - 2 ? phi : 0.0;
+    q[Q_T2] = iy == ly - 3 ? phi : 0.0;
+    const bool finite = isfinite(rho) && isfinite(usq) && isfinite(phi);
+    q[Q_MAXUSQ] = finite ? usq : 0.0;
+    q[Q_NONFINITE] = finite ? 0.0 : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Collision of one cell, f and g (src/collision.cu:22-32 and :74-81).
+// ---------------------------------------------------------------------------
+
+template <class P>
+__device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::T (&g)[Q],
+                                             typename P::T rho, typename P::T ux, typename P::T uy,
+                                             typename P::T uz, typename P::T phi, typename P::T F,
+                                             const StepArgs &a) {
+    using T = typename P::T;
+    if constexpr (P::exact) {
+        // Reference operation order, FP64.
+        const double RT = 1.0 / 3.0;
+        const double tau = a.tau, tauc = a.tauc;
+        const double uv = (ux * ux + uy * uy + uz * uz) / RT;
+        const double c1F = a.c1 * F;
+        // ((1 - 0.5/tau) * F_y * (c_y - v)) / RT for c_y = -1, 0, +1
+        const double pf[3] = {(c1F * (-1.0 - uy)) / RT, (c1F * (0.0 - uy)) / RT, (c1F * (1.0 - uy)) / RT};
+        double eu[Q];
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (i == 0) {
+                eu[0] = 0.0;
+            } else if constexpr (is_canonical(i)) {
+                eu[i] = cdotu<i>(ux, uy, uz) / RT;
+            } else {
+                eu[i] = -eu[OPP[i]];
+            }
+        });
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr double w = weight(i);
+            const double e = eu[i];
+            const double A = 0.5 * (e * e - uv);
+            const double feq = w * ((rho + e) + A);
+            const double feq0 = w * ((1.0 + e) + A);
+            double fc = -(f[i] - feq) / tau;
+            fc = fc + pf[CY[i] + 1] * feq0;
+            f[i] = f[i] + fc;
+            const double geq = (w * phi) * ((1.0 + e) + A);
+            g[i] = g[i] + (-(g[i] - geq) / tauc);
+        });
+    } else {
+        // FP32 arithmetic with reciprocals; same model.
+        const float omega = static_cast<float>(1.0 / a.tau);
+        const float omegac = static_cast<float>(1.0 / a.tauc);
+        const float uv = (ux * ux + uy * uy + uz * uz) * 3.0f;
+        const float c1F3 = static_cast<float>(a.c1) * F * 3.0f;
+        const float pf[3] = {c1F3 * (-1.0f - uy), c1F3 * (0.0f - uy), c1F3 * (1.0f - uy)};
+        float eu[Q];
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (i == 0) {
+                eu[0] = 0.0f;
+            } else if constexpr (is_canonical(i)) {
+                eu[i] = cdotu<i>(ux, uy, uz) * 3.0f;
+            } else {
+                eu[i] = -eu[OPP[i]];
+            }
+        });
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr float w = static_cast<float>(weight(i));
+            const float e = eu[i];
+            const float A = 0.5f * (e * e - uv);
+            const float feq = w * ((rho + e) + A);
+            const float feq0 = w * ((1.0f + e) + A);
+            f[i] = f[i] - (f[i] - feq) * omega + pf[CY[i] + 1] * feq0;
+            const float geq = (w * phi) * ((1.0f + e) + A);
+            g[i] = g[i] - (g[i] - geq) * omegac;
+        });
+    }
+    (void)sizeof(T);
+}
+
+// ---------------------------------------------------------------------------
+// The step kernel.
+// ---------------------------------------------------------------------------
+
+template <class P, int MODE, bool REDUCE, int BX>
+__global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
+    using S = typename P::S;
+    using T = typename P::T;
+    using L = typename P::L;
+
+    const int lx = a.lx, ly = a.ly, lz = a.lz;
+    const uint32_t b = blockIdx.x;
+    const uint32_t row = b / static_cast<uint32_t>(a.ntx);
+    const int bxi = static_cast<int>(b - row * static_cast<uint32_t>(a.ntx));
+    const int iy = static_cast<int>(row % static_cast<uint32_t>(ly));
+    const int iz = static_cast<int>(row / static_cast<uint32_t>(ly));
+    const int ix = bxi * BX + static_cast<int>(threadIdx.x);
+    const bool active = ix < lx;
+
+    double q[NQ];
+    if constexpr (REDUCE) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    }
+
+    if (active) {
+        const uint64_t n = a.n;
+        const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
+        const S *__restrict__ fa = static_cast<const S *>(a.fa);
+        const S *__restrict__ ga = static_cast<const S *>(a.ga);
+        S *__restrict__ fb = static_cast<S *>(a.fb);
+        S *__restrict__ gb = static_cast<S *>(a.gb);
+        const T t0 = static_cast<T>(a.t0);
+
+        T f[Q], g[Q];
+        T rho, ux, uy, uz, phi;
+        const bool wall_lo = iy == 0, wall_hi = iy == ly - 1;
+        const uint64_t wall_idx = (static_cast<uint64_t>(wall_hi ? 1 : 0) * lz + iz) * lx + ix;
+        L *__restrict__ wall_rho = static_cast<L *>(a.wall_rho);
+        L *__restrict__ lag = static_cast<L *>(a.lag);
+
+        if constexpr (MODE == MODE_PROLOGUE) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                f[i] = P::dec_f(fa[i * n + idx]);
+                g[i] = P::template dec_g<i>(ga[i * n + idx], t0);
+            });
+            rho = static_cast<T>(a.rho[idx]);
+            ux = static_cast<T>(a.ux[idx]);
+            uy = static_cast<T>(a.uy[idx]);
+            uz = static_cast<T>(a.uz[idx]);
+            phi = static_cast<T>(a.phi[idx]);
+        } else {
+            // ---- gather (pull) with the wall rules -------------------------
+            const bool xlo = ix > 0, xhi = ix < lx - 1;
+            const bool ylo = iy > 0, yhi = iy < ly - 1;
+            const bool zlo = iz > 0, zhi = iz < lz - 1;
+            const int sy = lx, sz = lx * ly;
+            T rw = T(0);
+            if (wall_lo || wall_hi) rw = static_cast<T>(wall_rho[wall_idx]);
+            const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
+                if constexpr (i == 0) {
+                    f[0] = P::dec_f(fa[idx]);
+                    g[0] = P::template dec_g<0>(ga[idx], t0);
+                } else {
+                    const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
+                    const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
+                    const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+                    if (inx && iny && inz) {
+                        const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
+                        f[i] = P::dec_f(fa[i * n + src]);
+                        g[i] = P::template dec_g<i>(ga[i * n + src], t0);
+                    } else {
+                        // Half-way bounce-back from the cell's own opposite
+                        // population (f at every wall; g at the x and z walls).
+                        f[i] = P::dec_f(fa[o * n + idx]);
+                        T gv = P::template dec_g<o>(ga[o * n + idx], t0);
+                        if constexpr (cy != 0) {
+                            // Isothermal anti-bounce-back at the y walls; the
+                            // reference applies the z walls after the y walls,
+                            // so an out-of-domain z source keeps bounce-back.
+                            if (!iny && inz) {
+                                constexpr T w2 = static_cast<T>(2.0 * weight(i));
+                                gv = -gv + w2 * tw * (T(1) + rw);
+                            }
+                        }
+                        g[i] = gv;
+                    }
+                }
+            });
+            // ---- moments, summed in direction order (src/macrovar.cu:17-27)
+            T jx = T(0), jy = T(0), jz = T(0);
+            rho = T(0);
+            phi = T(0);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                rho = rho + f[i];
+                if constexpr (CX[i] > 0) jx = jx + f[i];
+                if constexpr (CX[i] < 0) jx = jx - f[i];
+                if constexpr (CY[i] > 0) jy = jy + f[i];
+                if constexpr (CY[i] < 0) jy = jy - f[i];
+                if constexpr (CZ[i] > 0) jz = jz + f[i];
+                if constexpr (CZ[i] < 0) jz = jz - f[i];
+                phi = phi + g[i];
+            });
+            const T phi_prev = static_cast<T>(lag[idx]);
+            const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
+            ux = jx;
+            uy = jy + T(0.5) * fprev;
+            uz = jz;
+            if constexpr (MODE == MODE_FINALIZE) {
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    fb[i * n + idx] = P::enc_f(f[i]);
+                    gb[i * n + idx] = P::template enc_g<i>(g[i], t0);
+                });
+                a.rho[idx] = static_cast<double>(rho);
+                a.ux[idx] = static_cast<double>(ux);
+                a.uy[idx] = static_cast<double>(uy);
+                a.uz[idx] = static_cast<double>(uz);
+                a.phi[idx] = static_cast<double>(phi);
+                a.fy[idx] = static_cast<double>(fprev);
+            }
+        }
+
+        if constexpr (MODE != MODE_FINALIZE) {
+            // ---- forcing (src/macrovar.cu:45) and collision -----------------
+            const T F = static_cast<T>(a.grav_beta) * (phi - t0);
+            collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                fb[i * n + idx] = P::enc_f(f[i]);
+                gb[i * n + idx] = P::template enc_g<i>(g[i], t0);
+            });
+            lag[idx] = static_cast<L>(phi);
+            if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);
+        }
+
+        if constexpr (REDUCE && MODE != MODE_PROLOGUE) {
+            diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux),
+                       static_cast<double>(uy), static_cast<double>(uz), static_cast<double>(phi));
+        }
+    }
+
+    if constexpr (REDUCE && MODE != MODE_PROLOGUE) block_reduce_diag<BX>(q, a.partials);
+}
+
+template <class P>
+hipError_t launch_step_impl(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s) {
+    const int64_t nb = step_grid_blocks(a.lx, a.ly, a.lz, bx);
+    if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<uint32_t>(nb));
+#define LBM_LAUNCH(BXV)                                                                              \
+    do {                                                                                             \
+        const dim3 block(BXV);                                                                       \
+        if (mode == MODE_PROLOGUE)                                                                   \
+            hipLaunchKernelGGL((step_kernel<P, MODE_PROLOGUE, false, BXV>), grid, block, 0, s, a);   \
+        else if (mode == MODE_FUSED && !reduce)                                                      \
+            hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, false, BXV>), grid, block, 0, s, a);      \
+        else if (mode == MODE_FUSED)                                                                 \
+            hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, true, BXV>), grid, block, 0, s, a);       \
+        else if (!reduce)                                                                            \
+            hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, false, BXV>), grid, block, 0, s, a);   \
+        else                                                                                         \
+            hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, true, BXV>), grid, block, 0, s, a);    \
+    } while (0)
+    switch (bx) {
+        case 64: LBM_LAUNCH(64); break;
+        case 128: LBM_LAUNCH(128); break;
+        case 256: LBM_LAUNCH(256); break;
+        case 512: LBM_LAUNCH(512); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef LBM_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace lbm
