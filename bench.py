@@ -84,7 +84,20 @@ def main() -> int:
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", type=int, default=3,
+                    help="steps per candidate for the launch-shape autotuner (0: defaults)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
+                         "the HBM-counter calibration run")
     args = ap.parse_args()
+
+    if args.copy_calibration > 0:
+        from lbm_ddf_amd import capi
+
+        nbytes = int(args.copy_calibration * (1 << 30))
+        gbps = capi.copy_bandwidth(nbytes, 5)
+        print(json.dumps({"copy_bytes": nbytes, "reps": 5, "gbps": round(gbps, 1)}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
@@ -103,7 +116,14 @@ def main() -> int:
     c.synchronize()
     log(f"warm-up {args.warmup} steps")
     t = time.perf_counter()
-    c.step(args.warmup)
+    tuned = None
+    warm = args.warmup
+    if args.autotune > 0:
+        tuned = c.autotune(args.autotune)
+        warm = max(0, warm - tuned["steps_taken"])
+        log(f"autotune: block_x={tuned['block_x']} flags={tuned['flags']} "
+            f"({tuned['ms_per_step']:.3f} ms/step, {tuned['steps_taken']} steps)")
+    c.step(warm)
     c.synchronize()
     log(f"warm-up done in {time.perf_counter() - t:.1f} s; timing {args.steps} steps")
 
@@ -114,6 +134,7 @@ def main() -> int:
     wall = time.perf_counter() - t0
     log(f"timed region {wall:.2f} s (events {kernel_ms / 1e3:.2f} s)")
 
+    c_tuning = c.tuning
     d = c.reduce()
     if d["nonfinite"] != 0:
         log("WARNING: non-finite values in the final state")
@@ -143,6 +164,7 @@ def main() -> int:
                         f"{prec} populations, fused collide+stream of f and g",
             "cells": n,
             "parallelism": "single GPU",
+            "launch": {"block_x": c_tuning[0], "flags": c_tuning[1], "autotuned": tuned is not None},
         },
         "roofline": {
             "bound": "hbm",

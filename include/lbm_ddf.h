@@ -159,6 +159,22 @@ int lbm_step_count(const lbm_ctx *ctx, int64_t *out);
 /* Wait for all work on the context's stream. */
 int lbm_synchronize(lbm_ctx *ctx);
 
+/* Launch-shape tuning of the fused pass (no reference counterpart; the
+ * reference fixes an 8x8x4 block, src/parameters.h:29-31).  block_x: threads
+ * per block along x (64, 128, 256 or 512); flags: bit 0 non-temporal stores,
+ * bit 1 non-temporal loads of unshifted populations, bit 2 three waves per
+ * SIMD.  Every setting computes the same values bit for bit. */
+int lbm_set_tuning(lbm_ctx *ctx, int32_t block_x, int32_t flags);
+int lbm_get_tuning(const lbm_ctx *ctx, int32_t *block_x, int32_t *flags);
+/* Times every valid (block_x, flags) on the live state for `steps` fused
+ * steps each and keeps the fastest.  Advances the simulation by the steps it
+ * runs (*steps_taken); results are unaffected by the choice. */
+int lbm_autotune(lbm_ctx *ctx, int32_t steps, int64_t *steps_taken, double *best_ms_per_step);
+/* Device-to-device copy bandwidth (read + write bytes per second / 1e9) of a
+ * plain copy kernel over two buffers of `bytes`: the achievable-bandwidth
+ * reference for the roofline fraction. */
+int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps);
+
 /* Host copies of the reference state (the D2H copies of src/output.cu).
  * Fields: LX*LY*LZ doubles; populations: 27*LX*LY*LZ doubles. */
 int lbm_get_field(lbm_ctx *ctx, int32_t field, double *host);

@@ -91,6 +91,10 @@ SIGNATURES = {
     "lbm_set_field": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
     "lbm_get_pops": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
     "lbm_set_pops": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+    "lbm_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "lbm_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "lbm_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _D]),
+    "lbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, _D]),
 }
 
 
@@ -136,6 +140,13 @@ def derive(params: Params) -> dict:
     d = Derived()
     _check(load().lbm_derive(ctypes.byref(params), ctypes.byref(d)))
     return {k: getattr(d, k) for k, _ in d._fields_}
+
+
+def copy_bandwidth(nbytes: int = 4 << 30, reps: int = 10) -> float:
+    """GB/s (read + write) of a plain device copy kernel."""
+    v = ctypes.c_double()
+    _check(load().lbm_copy_bandwidth(ctypes.c_uint64(int(nbytes)), int(reps), ctypes.byref(v)))
+    return v.value
 
 
 def bytes_required(params: Params) -> int:
@@ -225,6 +236,21 @@ class Cavity:
         nout = ctypes.c_int64()
         _check(self.lib.lbm_run_diag(self.h, int(n), int(every), arr, cap, ctypes.byref(nout)))
         return [arr[i].as_dict() for i in range(nout.value)]
+
+    def set_tuning(self, block_x: int, flags: int = 0):
+        _check(self.lib.lbm_set_tuning(self.h, int(block_x), int(flags)))
+
+    @property
+    def tuning(self) -> tuple:
+        bx, fl = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.lbm_get_tuning(self.h, ctypes.byref(bx), ctypes.byref(fl)))
+        return int(bx.value), int(fl.value)
+
+    def autotune(self, steps: int = 3) -> dict:
+        taken, best = ctypes.c_int64(), ctypes.c_double()
+        _check(self.lib.lbm_autotune(self.h, int(steps), ctypes.byref(taken), ctypes.byref(best)))
+        bx, fl = self.tuning
+        return {"block_x": bx, "flags": fl, "steps_taken": int(taken.value), "ms_per_step": best.value}
 
     def finalize(self):
         _check(self.lib.lbm_finalize(self.h))

@@ -82,6 +82,7 @@ struct lbm_ctx {
     int phase = PHASE_EMPTY;
     int64_t nstep = 0;
     int bx = 256;
+    int flags = 0;
     int device = 0;
 };
 
@@ -119,9 +120,9 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
 
 hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
     switch (c->prec) {
-        case LBM_PREC_FP64: return lbm::launch_step_f64(a, mode, reduce, c->bx, c->stream);
-        case LBM_PREC_FP16: return lbm::launch_step_f16(a, mode, reduce, c->bx, c->stream);
-        case LBM_PREC_BF16: return lbm::launch_step_bf16(a, mode, reduce, c->bx, c->stream);
+        case LBM_PREC_FP64: return lbm::launch_step_f64(a, mode, reduce, c->bx, c->flags, c->stream);
+        case LBM_PREC_FP16: return lbm::launch_step_f16(a, mode, reduce, c->bx, c->flags, c->stream);
+        case LBM_PREC_BF16: return lbm::launch_step_bf16(a, mode, reduce, c->bx, c->flags, c->stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -213,13 +214,23 @@ void derive(const lbm_params *p, lbm_derived *d, double *beta_out) {
     if (beta_out) *beta_out = beta;
 }
 
+// Partial-sum records for the largest grid any block width can launch.
+size_t max_partials_bytes(const lbm_params *p) {
+    return static_cast<size_t>(lbm::step_grid_blocks(p->lx, p->ly, p->lz, 64)) * lbm::NQ * 8;
+}
+
+bool valid_block(int lx, int bx) {
+    if (bx != 64 && bx != 128 && bx != 256 && bx != 512) return false;
+    return bx == 64 || bx / 2 < lx;  // no block more than half idle
+}
+
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     const uint64_t pops = 4ull * 27ull * n * elem_size(p->precision);
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + 2ull * p->lx * p->lz * side_size(p->precision);
-    const uint64_t part = static_cast<uint64_t>(lbm::step_grid_blocks(p->lx, p->ly, p->lz, bx)) * lbm::NQ * 8ull;
-    return pops + fields + side + part;
+    (void)bx;
+    return pops + fields + side + max_partials_bytes(p);
 }
 
 }  // namespace
@@ -298,7 +309,7 @@ int lbm_create(const lbm_params *p, lbm_ctx **out) {
         {reinterpret_cast<void **>(&c->fy), n * 8},
         {&c->lag, n * side_size(c->prec)},
         {&c->wall_rho, 2ull * p->lx * p->lz * side_size(c->prec)},
-        {reinterpret_cast<void **>(&c->partials), static_cast<size_t>(nblocks(c)) * lbm::NQ * 8},
+        {reinterpret_cast<void **>(&c->partials), max_partials_bytes(p)},
     };
     for (auto &al : allocs) {
         if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
@@ -496,6 +507,103 @@ int lbm_synchronize(lbm_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     LBM_HIP(hipStreamSynchronize(c->stream));
     return LBM_OK;
+}
+
+// ---- tuning ---------------------------------------------------------------
+
+int lbm_set_tuning(lbm_ctx *c, int32_t block_x, int32_t flags) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!valid_block(c->p.lx, block_x)) return fail(LBM_E_INVALID, "block_x must be 64, 128, 256 or 512 and fit LX");
+    if (flags < 0 || flags > 7) return fail(LBM_E_INVALID, "flags must be in 0..7");
+    if (block_x != c->bx && c->phase == PHASE_FUSED) {
+        // partial-sum layouts differ per block width only for reductions,
+        // which are finished within one call; switching is safe at any time.
+    }
+    c->bx = block_x;
+    c->flags = flags;
+    return LBM_OK;
+}
+
+int lbm_get_tuning(const lbm_ctx *c, int32_t *block_x, int32_t *flags) {
+    if (int rc = check_ctx(c)) return rc;
+    if (block_x) *block_x = c->bx;
+    if (flags) *flags = c->flags;
+    return LBM_OK;
+}
+
+int lbm_autotune(lbm_ctx *c, int32_t steps, int64_t *steps_taken, double *best_ms) {
+    if (int rc = check_ctx(c)) return rc;
+    if (steps < 1) return fail(LBM_E_INVALID, "steps must be >= 1");
+    if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
+    int64_t taken = 0;
+    if (c->phase == PHASE_REF) {
+        if (int rc = lbm_step(c, 1)) return rc;
+        taken += 1;
+    }
+    const int blocks[] = {64, 128, 256, 512};
+    int best_bx = c->bx, best_fl = c->flags;
+    double best = 1e300;
+    for (int bx : blocks) {
+        if (!valid_block(c->p.lx, bx)) continue;
+        for (int fl = 0; fl < 8; ++fl) {
+            c->bx = bx;
+            c->flags = fl;
+            // one untimed launch per candidate (code object load, caches)
+            if (int rc = lbm_step(c, 1)) return rc;
+            double ms = 0.0;
+            if (int rc = lbm_step_timed(c, steps, &ms)) return rc;
+            taken += 1 + steps;
+            if (ms < best) {
+                best = ms;
+                best_bx = bx;
+                best_fl = fl;
+            }
+        }
+    }
+    c->bx = best_bx;
+    c->flags = best_fl;
+    if (steps_taken) *steps_taken = taken;
+    if (best_ms) *best_ms = best / steps;
+    return LBM_OK;
+}
+
+int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps) {
+    if (!gbps || reps < 1 || bytes < 4096) return fail(LBM_E_INVALID, "bad arguments");
+    bytes &= ~static_cast<uint64_t>(4095);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LBM_OK;
+    float ms = 0.0f;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        rc = fail(LBM_E_NOMEM, "copy buffers");
+        goto done;
+    }
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(LBM_E_HIP, "stream/event creation");
+        goto done;
+    }
+    if (hipMemsetAsync(a, 0, bytes, st) != hipSuccess || lbm::launch_copy(a, b, bytes, st) != hipSuccess) {
+        rc = fail(LBM_E_HIP, "copy warm-up");
+        goto done;
+    }
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r) (void)lbm::launch_copy(r % 2 ? b : a, r % 2 ? a : b, bytes, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(LBM_E_HIP, "copy timing");
+        goto done;
+    }
+    *gbps = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
+done:
+    if (st) (void)hipStreamSynchronize(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 // ---- host copies ----------------------------------------------------------

@@ -57,9 +57,9 @@ struct DiagArgs {
 };
 
 // Launchers (defined in step_f64.hip / step_lp.hip / stage_f64.hip / reduce.hip).
-hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
-hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
-hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s);
+hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
+hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
+hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
 
 // Reference-state stage kernels (FP64).
 hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
@@ -77,6 +77,9 @@ hipError_t launch_reduce_fields(const StepArgs &a, int bx, hipStream_t s);
 hipError_t launch_diag_finish(const double *partials, int64_t nblocks, const DiagArgs &d,
                               lbm_diag *dst, hipStream_t s);
 hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s);
+
+// Plain device copy of bytes (multiple of 16) for the bandwidth reference.
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 
 // Population plane conversion between the storage format and FP64.
 hipError_t launch_decode_plane(int precision, int which, int ip, const void *src, double *dst,

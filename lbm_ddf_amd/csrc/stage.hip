@@ -370,6 +370,13 @@ hipError_t codec_plane(bool encode, int which, int ip, const void *src, void *ds
     return err;
 }
 
+// Plain device copy, 16 bytes per thread per iteration, grid-stride: the
+// achievable-bandwidth reference and the calibration kernel for HBM counters.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
 hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
@@ -435,6 +442,13 @@ hipError_t launch_diag_finish(const double *partials, int64_t nblocks, const Dia
 
 hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s) {
     hipLaunchKernelGGL(profile_kernel, dim3(a.ly), dim3(kProfileThreads), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s) {
+    const uint64_t n16 = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(256 * 32), dim3(256), 0, s, static_cast<const uint4 *>(src),
+                       static_cast<uint4 *>(dst), n16);
     return hipGetLastError();
 }
 

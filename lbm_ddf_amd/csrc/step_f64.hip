@@ -5,8 +5,8 @@
 
 namespace lbm {
 
-hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s) {
-    return launch_step_impl<PolicyF64>(a, mode, reduce, bx, s);
+hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s) {
+    return launch_step_impl<PolicyF64>(a, mode, reduce, bx, flags, s);
 }
 
 }  // namespace lbm

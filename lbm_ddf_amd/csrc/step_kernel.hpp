@@ -236,11 +236,35 @@ __device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::
 }
 
 // ---------------------------------------------------------------------------
+// Launch variants.  All variants compute identical values; they differ only in
+// cache hints and occupancy, and the autotuner picks among them.
+// ---------------------------------------------------------------------------
+enum StepFlags : int {
+    kNtStore = 1,  // non-temporal stores of the new populations (streamed out once)
+    kNtLoad = 2,   // non-temporal loads of populations without an x shift
+    kWaves3 = 4,   // ask the register allocator for 3 waves per SIMD
+};
+
+template <bool NT, typename S>
+__device__ __forceinline__ S ld(const S *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename S>
+__device__ __forceinline__ void st(S *p, S v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel.
 // ---------------------------------------------------------------------------
 
-template <class P, int MODE, bool REDUCE, int BX>
-__global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
+template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu((FLAGS & kWaves3) ? 3 : 1)))
+void step_kernel(const StepArgs a) {
+    constexpr bool NTS = (FLAGS & kNtStore) != 0;
+    constexpr bool NTL = (FLAGS & kNtLoad) != 0;
     using S = typename P::S;
     using T = typename P::T;
     using L = typename P::L;
@@ -279,8 +303,8 @@ __global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
         if constexpr (MODE == MODE_PROLOGUE) {
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
-                f[i] = P::dec_f(fa[i * n + idx]);
-                g[i] = P::template dec_g<i>(ga[i * n + idx], t0);
+                f[i] = P::dec_f(ld<NTL>(&fa[i * n + idx]));
+                g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), t0);
             });
             rho = static_cast<T>(a.rho[idx]);
             ux = static_cast<T>(a.ux[idx]);
@@ -300,16 +324,16 @@ __global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
                 constexpr int i = I;
                 constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
                 if constexpr (i == 0) {
-                    f[0] = P::dec_f(fa[idx]);
-                    g[0] = P::template dec_g<0>(ga[idx], t0);
+                    f[0] = P::dec_f(ld<NTL>(&fa[idx]));
+                    g[0] = P::template dec_g<0>(ld<NTL>(&ga[idx]), t0);
                 } else {
                     const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                     const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
                     const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
                     if (inx && iny && inz) {
                         const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
-                        f[i] = P::dec_f(fa[i * n + src]);
-                        g[i] = P::template dec_g<i>(ga[i * n + src], t0);
+                        f[i] = P::dec_f(ld<NTL && cx == 0>(&fa[i * n + src]));
+                        g[i] = P::template dec_g<i>(ld<NTL && cx == 0>(&ga[i * n + src]), t0);
                     } else {
                         // Half-way bounce-back from the cell's own opposite
                         // population (f at every wall; g at the x and z walls).
@@ -351,8 +375,8 @@ __global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
             if constexpr (MODE == MODE_FINALIZE) {
                 static_for<Q>([&](auto I) {
                     constexpr int i = I;
-                    fb[i * n + idx] = P::enc_f(f[i]);
-                    gb[i * n + idx] = P::template enc_g<i>(g[i], t0);
+                    st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
+                    st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
                 });
                 a.rho[idx] = static_cast<double>(rho);
                 a.ux[idx] = static_cast<double>(ux);
@@ -369,8 +393,8 @@ __global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
             collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
-                fb[i * n + idx] = P::enc_f(f[i]);
-                gb[i * n + idx] = P::template enc_g<i>(g[i], t0);
+                st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
+                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
             });
             lag[idx] = static_cast<L>(phi);
             if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);
@@ -385,34 +409,52 @@ __global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
     if constexpr (REDUCE && MODE != MODE_PROLOGUE) block_reduce_diag<BX>(q, a.partials);
 }
 
+// Variant flags used by the plain fused pass; the other passes use FLAGS=0.
+template <class P, int BXV, int FL>
+hipError_t launch_fused_variant(const StepArgs &a, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, false, BXV, FL>), grid, dim3(BXV), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class P, int BXV>
+hipError_t launch_bx(const StepArgs &a, int mode, bool reduce, int flags, dim3 grid, hipStream_t s) {
+    if (mode == MODE_FUSED && !reduce) {
+        switch (flags) {
+            case 0: return launch_fused_variant<P, BXV, 0>(a, grid, s);
+            case 1: return launch_fused_variant<P, BXV, 1>(a, grid, s);
+            case 2: return launch_fused_variant<P, BXV, 2>(a, grid, s);
+            case 3: return launch_fused_variant<P, BXV, 3>(a, grid, s);
+            case 4: return launch_fused_variant<P, BXV, 4>(a, grid, s);
+            case 5: return launch_fused_variant<P, BXV, 5>(a, grid, s);
+            case 6: return launch_fused_variant<P, BXV, 6>(a, grid, s);
+            case 7: return launch_fused_variant<P, BXV, 7>(a, grid, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    const dim3 block(BXV);
+    if (mode == MODE_PROLOGUE)
+        hipLaunchKernelGGL((step_kernel<P, MODE_PROLOGUE, false, BXV, 0>), grid, block, 0, s, a);
+    else if (mode == MODE_FUSED)
+        hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, true, BXV, 0>), grid, block, 0, s, a);
+    else if (!reduce)
+        hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, false, BXV, 0>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, true, BXV, 0>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
 template <class P>
-hipError_t launch_step_impl(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s) {
+hipError_t launch_step_impl(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s) {
     const int64_t nb = step_grid_blocks(a.lx, a.ly, a.lz, bx);
     if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(nb));
-#define LBM_LAUNCH(BXV)                                                                              \
-    do {                                                                                             \
-        const dim3 block(BXV);                                                                       \
-        if (mode == MODE_PROLOGUE)                                                                   \
-            hipLaunchKernelGGL((step_kernel<P, MODE_PROLOGUE, false, BXV>), grid, block, 0, s, a);   \
-        else if (mode == MODE_FUSED && !reduce)                                                      \
-            hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, false, BXV>), grid, block, 0, s, a);      \
-        else if (mode == MODE_FUSED)                                                                 \
-            hipLaunchKernelGGL((step_kernel<P, MODE_FUSED, true, BXV>), grid, block, 0, s, a);       \
-        else if (!reduce)                                                                            \
-            hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, false, BXV>), grid, block, 0, s, a);   \
-        else                                                                                         \
-            hipLaunchKernelGGL((step_kernel<P, MODE_FINALIZE, true, BXV>), grid, block, 0, s, a);    \
-    } while (0)
     switch (bx) {
-        case 64: LBM_LAUNCH(64); break;
-        case 128: LBM_LAUNCH(128); break;
-        case 256: LBM_LAUNCH(256); break;
-        case 512: LBM_LAUNCH(512); break;
+        case 64: return launch_bx<P, 64>(a, mode, reduce, flags, grid, s);
+        case 128: return launch_bx<P, 128>(a, mode, reduce, flags, grid, s);
+        case 256: return launch_bx<P, 256>(a, mode, reduce, flags, grid, s);
+        case 512: return launch_bx<P, 512>(a, mode, reduce, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
-#undef LBM_LAUNCH
-    return hipGetLastError();
 }
 
 }  // namespace lbm

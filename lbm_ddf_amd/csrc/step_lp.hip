@@ -4,12 +4,12 @@
 
 namespace lbm {
 
-hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s) {
-    return launch_step_impl<PolicyF16>(a, mode, reduce, bx, s);
+hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s) {
+    return launch_step_impl<PolicyF16>(a, mode, reduce, bx, flags, s);
 }
 
-hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, hipStream_t s) {
-    return launch_step_impl<PolicyBF16>(a, mode, reduce, bx, s);
+hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s) {
+    return launch_step_impl<PolicyBF16>(a, mode, reduce, bx, flags, s);
 }
 
 }  // namespace lbm

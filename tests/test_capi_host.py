@@ -1,0 +1,85 @@
+"""Host-side checks of the C ABI (CPU only: no call touches a GPU).
+
+The library must load, export every function declared in include/lbm_ddf.h,
+and its host-only entry points (parameter derivation, sizing, argument
+validation) must agree with the oracle and the reference's formulas.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+import oracle
+from lbm_ddf_amd import capi
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "lbm_ddf.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes binding covers them all
+    assert set(names) == set(capi.SIGNATURES)
+
+
+@pytest.mark.parametrize("L,ra,pr", [(64, 1e5, 0.71), (512, 1e7, 0.71), (512, 1e8, 0.71), (37, 3e4, 4.4)])
+def test_derive_matches_oracle(L, ra, pr):
+    d = capi.derive(capi.make_params(L, L, L, rayleigh=ra, prandtl=pr, t_hot=0.5, t_cold=-0.5))
+    o = oracle.make_params(L, L, L, rayleigh=ra, prandtl=pr, t_hot=0.5, t_cold=-0.5)
+    for k in ("visc", "tau", "u0", "diff", "tauc", "t0"):
+        assert d[k] == getattr(o, k), k
+
+
+def test_reference_default_parameters():
+    """src/parameters.h: 400^3, Ra=1e7, Pr=0.71, T_HOT=1, T_COLD=0, g=1."""
+    d = capi.derive(capi.make_params(400, 400, 400))
+    assert d["t0"] == 0.5
+    assert d["tau"] == pytest.approx(3 * 400 * (0.02 * 0.71 / 1e7) ** 0.5 + 0.5, rel=1e-14)
+
+
+def test_bytes_required_fp64_512():
+    b = capi.bytes_required(capi.make_params(512, 512, 512))
+    n = 512 ** 3
+    assert b >= 4 * 27 * n * 8 + 6 * n * 8
+    assert b < 125e9  # fits one 288 GB MI355X with room to spare
+    b16 = capi.bytes_required(capi.make_params(512, 512, 512, precision="fp16"))
+    assert b16 < b / 2.5
+
+
+@pytest.mark.parametrize("shape", [(4, 2, 4), (0, 8, 8), (8, 8, 0)])
+def test_create_rejects_bad_geometry_without_touching_the_gpu(shape):
+    lib = capi.load()
+    h = ctypes.c_void_p()
+    p = capi.make_params(*shape)
+    rc = lib.lbm_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1
+    assert b"ly >= 3" in lib.lbm_last_error()
+    assert not h.value
+
+
+def test_create_rejects_bad_precision_and_physics():
+    lib = capi.load()
+    h = ctypes.c_void_p()
+    p = capi.make_params(8, 8, 8)
+    p.precision = 7
+    assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    p = capi.make_params(8, 8, 8, rayleigh=-1.0)
+    assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"Ra" in lib.lbm_last_error()
+
+
+def test_null_context_is_an_error_not_a_crash():
+    lib = capi.load()
+    assert lib.lbm_step(None, 1) == -1
+    assert lib.lbm_finalize(None) == -1
+    assert lib.lbm_destroy(None) == 0

@@ -166,7 +166,11 @@ def test_profile_matches_oracle():
     c, o = make_pair(14, 10, 12, seed=9, amp=1.0e-2, rayleigh=1.0e6)
     c.step(30)
     o.run(30)
-    np.testing.assert_allclose(c.profile(), o.profile(), rtol=1e-11, atol=1e-14)
+    g, r = c.profile(), o.profile()
+    # plane means: summation order differs (tree vs serial)
+    np.testing.assert_allclose(g[:, :4], r[:, :4], rtol=1e-11, atol=1e-14)
+    # T_rms = sqrt(<phi^2> - <phi>^2) cancels: compare against its scale
+    np.testing.assert_allclose(g[:, 4], r[:, 4], rtol=0, atol=1e-12 * np.max(np.abs(r[:, 3])) / np.min(r[:, 4]))
 
 
 def test_pops_roundtrip_fp64():
