@@ -66,6 +66,7 @@ struct lbm_ctx {
     lbm_derived d{};
     double beta = 0.0;
     uint64_t n = 0;
+    uint64_t ps = 0;  // population plane stride (elements)
     int prec = 0;
     void *f[2] = {nullptr, nullptr};
     void *g[2] = {nullptr, nullptr};
@@ -95,6 +96,7 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
     a.fb = fb;
     a.gb = gb;
     a.n = c->n;
+    a.ps = c->ps;
     a.lx = c->p.lx;
     a.ly = c->p.ly;
     a.lz = c->p.lz;
@@ -226,7 +228,7 @@ bool valid_block(int lx, int bx) {
 
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
-    const uint64_t pops = 4ull * 27ull * n * elem_size(p->precision);
+    const uint64_t pops = 4ull * 27ull * lbm::plane_stride(n) * elem_size(p->precision);
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + 2ull * p->lx * p->lz * side_size(p->precision);
     (void)bx;
@@ -277,6 +279,7 @@ int lbm_create(const lbm_params *p, lbm_ctx **out) {
     c->p = *p;
     c->prec = p->precision;
     c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
+    c->ps = lbm::plane_stride(c->n);
     derive(p, &c->d, &c->beta);
     c->bx = default_block(p->lx);
     hipError_t e = hipGetDevice(&c->device);
@@ -292,7 +295,7 @@ int lbm_create(const lbm_params *p, lbm_ctx **out) {
         return hip_fail(e, "stream/event creation");
     }
     const uint64_t n = c->n;
-    const size_t pop_bytes = 27ull * n * elem_size(c->prec);
+    const size_t pop_bytes = 27ull * c->ps * elem_size(c->prec);
     struct Alloc {
         void **ptr;
         size_t bytes;
@@ -645,7 +648,8 @@ int lbm_get_pops(lbm_ctx *c, int32_t which, double *host) {
     if (int rc = ensure_ref(c)) return rc;
     void *src = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
     if (c->prec == LBM_PREC_FP64) {
-        LBM_HIP(hipMemcpyAsync(host, src, sizeof(double) * 27 * c->n, hipMemcpyDeviceToHost, c->stream));
+        LBM_HIP(hipMemcpy2DAsync(host, sizeof(double) * c->n, src, sizeof(double) * c->ps, sizeof(double) * c->n, 27,
+                                 hipMemcpyDeviceToHost, c->stream));
         LBM_HIP(hipStreamSynchronize(c->stream));
         return LBM_OK;
     }
@@ -653,8 +657,8 @@ int lbm_get_pops(lbm_ctx *c, int32_t which, double *host) {
         return fail(LBM_E_NOMEM, "plane staging buffer");
     const size_t es = elem_size(c->prec);
     for (int ip = 0; ip < 27; ++ip) {
-        const void *ps = static_cast<const char *>(src) + static_cast<size_t>(ip) * c->n * es;
-        LBM_HIP(lbm::launch_decode_plane(c->prec, which, ip, ps, c->plane, c->n, c->d.t0, c->stream));
+        const void *pl = static_cast<const char *>(src) + static_cast<size_t>(ip) * c->ps * es;
+        LBM_HIP(lbm::launch_decode_plane(c->prec, which, ip, pl, c->plane, c->n, c->d.t0, c->stream));
         LBM_HIP(hipMemcpyAsync(host + static_cast<size_t>(ip) * c->n, c->plane, sizeof(double) * c->n,
                                hipMemcpyDeviceToHost, c->stream));
     }
@@ -668,7 +672,8 @@ int lbm_set_pops(lbm_ctx *c, int32_t which, const double *host) {
     if (int rc = ensure_ref(c)) return rc;
     void *dst = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
     if (c->prec == LBM_PREC_FP64) {
-        LBM_HIP(hipMemcpyAsync(dst, host, sizeof(double) * 27 * c->n, hipMemcpyHostToDevice, c->stream));
+        LBM_HIP(hipMemcpy2DAsync(dst, sizeof(double) * c->ps, host, sizeof(double) * c->n, sizeof(double) * c->n, 27,
+                                 hipMemcpyHostToDevice, c->stream));
         LBM_HIP(hipStreamSynchronize(c->stream));
         return LBM_OK;
     }
@@ -676,7 +681,7 @@ int lbm_set_pops(lbm_ctx *c, int32_t which, const double *host) {
         return fail(LBM_E_NOMEM, "plane staging buffer");
     const size_t es = elem_size(c->prec);
     for (int ip = 0; ip < 27; ++ip) {
-        void *pd = static_cast<char *>(dst) + static_cast<size_t>(ip) * c->n * es;
+        void *pd = static_cast<char *>(dst) + static_cast<size_t>(ip) * c->ps * es;
         LBM_HIP(hipMemcpyAsync(c->plane, host + static_cast<size_t>(ip) * c->n, sizeof(double) * c->n,
                                hipMemcpyHostToDevice, c->stream));
         LBM_HIP(lbm::launch_encode_plane(c->prec, which, ip, c->plane, pd, c->n, c->d.t0, c->stream));

@@ -35,7 +35,8 @@ enum DiagQ : int {
 struct StepArgs {
     const void *fa, *ga;  // source populations [27][n]
     void *fb, *gb;        // destination populations [27][n]
-    uint64_t n;           // cells per population plane
+    uint64_t n;           // cells
+    uint64_t ps;          // population plane stride in elements (>= n, staggered)
     int32_t lx, ly, lz;
     int32_t ntx;          // blocks per x-row
     double tau, tauc;     // relaxation times (src/globals.cu:14,17)
@@ -86,6 +87,13 @@ hipError_t launch_decode_plane(int precision, int which, int ip, const void *src
                                uint64_t n, double t0, hipStream_t s);
 hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst,
                                uint64_t n, double t0, hipStream_t s);
+
+// Population planes are not packed back to back: each plane starts 64
+// elements (512 B in FP64, 128 B in 16-bit) later in its HBM interleave than
+// the previous one, so the 54 concurrent streams of a step do not all hit the
+// same channel and bank at the same offset.  Measured on MI355X: +2.5% (FP64)
+// and +12% (16-bit) over tightly packed 2^k-cell planes.
+inline uint64_t plane_stride(uint64_t n) { return ((n + 63) / 64) * 64 + 64; }
 
 inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
     int64_t ntx = (lx + bx - 1) / bx;

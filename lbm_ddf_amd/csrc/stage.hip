@@ -48,7 +48,7 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
         const double t = a.t_hot - (a.t_hot - a.t_cold) * iy / (a.ly - 1.0);
         phi = t + perturbation(seed, idx, amp);
     }
-    const uint64_t n = a.n;
+    const uint64_t n = a.ps;  // population plane stride
     a.rho[idx] = 0.0;
     a.ux[idx] = 0.0;
     a.uy[idx] = 0.0;
@@ -81,7 +81,7 @@ __global__ void forcing_kernel(StepArgs a) {
 __global__ void collide_f_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.n;
+    const uint64_t n = a.ps;  // population plane stride
     double *f = static_cast<double *>(a.fb);
     const double fy9 = a.fy[idx], rho9 = a.rho[idx];
     const double u9 = a.ux[idx], v9 = a.uy[idx], w9 = a.uz[idx];
@@ -103,7 +103,7 @@ __global__ void collide_f_kernel(StepArgs a) {
 __global__ void collide_g_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.n;
+    const uint64_t n = a.ps;  // population plane stride
     double *g = static_cast<double *>(a.gb);
     const double phi9 = a.phi[idx];
     const double u9 = a.ux[idx], v9 = a.uy[idx], w9 = a.uz[idx];
@@ -129,7 +129,7 @@ __global__ void stream_kernel(StepArgs a) {
     int ix, iy, iz;
     cell_coords(idx, a.lx, a.ly, ix, iy, iz);
     const int lx = a.lx, ly = a.ly, lz = a.lz;
-    const uint64_t n = a.n;
+    const uint64_t n = a.ps;  // population plane stride
     const double *src = static_cast<const double *>(SCALAR ? a.ga : a.fa);
     double *dst = static_cast<double *>(SCALAR ? a.gb : a.fb);
     static_for<Q>([&](auto I) {
@@ -190,7 +190,7 @@ __global__ void stream_kernel(StepArgs a) {
 __global__ void macrovar_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.n;
+    const uint64_t n = a.ps;  // population plane stride
     const double *f = static_cast<const double *>(a.fa);
     const double *g = static_cast<const double *>(a.ga);
     double rho = 0.0, ux = 0.0, uy = 0.0, uz = 0.0, phi = 0.0;

@@ -285,7 +285,7 @@ void step_kernel(const StepArgs a) {
     }
 
     if (active) {
-        const uint64_t n = a.n;
+        const uint64_t n = a.ps;  // population plane stride
         const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
         const S *__restrict__ fa = static_cast<const S *>(a.fa);
         const S *__restrict__ ga = static_cast<const S *>(a.ga);
