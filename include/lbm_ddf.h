@@ -162,8 +162,8 @@ int lbm_synchronize(lbm_ctx *ctx);
 /* Launch-shape tuning of the fused pass (no reference counterpart; the
  * reference fixes an 8x8x4 block, src/parameters.h:29-31).  block_x: threads
  * per block along x (64, 128, 256 or 512); flags: bit 0 non-temporal stores,
- * bit 1 non-temporal loads of unshifted populations, bit 2 three waves per
- * SIMD.  Every setting computes the same values bit for bit. */
+ * bit 1 non-temporal loads of unshifted populations, bit 2 XCD-aware block
+ * order.  Every setting computes the same values bit for bit. */
 int lbm_set_tuning(lbm_ctx *ctx, int32_t block_x, int32_t flags);
 int lbm_get_tuning(const lbm_ctx *ctx, int32_t *block_x, int32_t *flags);
 /* Times every valid (block_x, flags) on the live state for `steps` fused

@@ -242,8 +242,21 @@ __device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::
 enum StepFlags : int {
     kNtStore = 1,  // non-temporal stores of the new populations (streamed out once)
     kNtLoad = 2,   // non-temporal loads of populations without an x shift
-    kWaves3 = 4,   // ask the register allocator for 3 waves per SIMD
+    kXcdSwizzle = 4,  // XCD-aware block order (see xcd_swizzle)
 };
+
+// The dispatcher hands workgroups to the 8 XCDs of an MI355X round-robin
+// (block b runs on XCD b % 8), and each XCD has its own L2.  Adjacent
+// segments of one x-row share the cache lines that the x-shifted loads
+// straddle; without remapping those lines are fetched once per XCD.  This
+// bijection on [0, nb) gives each XCD a contiguous range of logical blocks,
+// so a row's segments run on one XCD, close together in time.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    const uint32_t xcd = b % kXcds, k = b / kXcds;
+    return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
 
 template <bool NT, typename S>
 __device__ __forceinline__ S ld(const S *p) {
@@ -261,8 +274,7 @@ __device__ __forceinline__ void st(S *p, S v) {
 // ---------------------------------------------------------------------------
 
 template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
-__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu((FLAGS & kWaves3) ? 3 : 1)))
-void step_kernel(const StepArgs a) {
+__global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
     using S = typename P::S;
@@ -270,7 +282,7 @@ void step_kernel(const StepArgs a) {
     using L = typename P::L;
 
     const int lx = a.lx, ly = a.ly, lz = a.lz;
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = (FLAGS & kXcdSwizzle) ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t row = b / static_cast<uint32_t>(a.ntx);
     const int bxi = static_cast<int>(b - row * static_cast<uint32_t>(a.ntx));
     const int iy = static_cast<int>(row % static_cast<uint32_t>(ly));
