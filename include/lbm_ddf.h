@@ -175,6 +175,41 @@ int lbm_autotune(lbm_ctx *ctx, int32_t steps, int64_t *steps_taken, double *best
  * reference for the roofline fraction. */
 int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps);
 
+/* ---- Ensembles: many cavities, one launch per step ------------------------
+ * No reference counterpart (the reference runs one cavity per process).
+ * Cavities of different size, Ra, Pr and walls advance together: one
+ * persistent launch per step over all resident cavities, step sequences
+ * replayed from a captured hipGraph.  Each cavity's result after k of its own
+ * steps equals a standalone lbm_step run of k steps bit for bit.  Cavities
+ * join (lbm_ensemble_add: conductive start) and leave (lbm_ensemble_remove)
+ * between calls.  A member context may be read through the single-cavity
+ * calls (lbm_get_field, lbm_reduce, ...) but not stepped on its own. */
+typedef struct lbm_ensemble lbm_ensemble;
+int lbm_ensemble_create(int32_t precision, lbm_ensemble **out);
+int lbm_ensemble_destroy(lbm_ensemble *ens);
+/* Allocates a cavity (params->precision must match) with the conductive
+ * start of lbm_init_conductive; *slot receives its index. */
+int lbm_ensemble_add(lbm_ensemble *ens, const lbm_params *params, uint64_t seed, double amp, int32_t *slot);
+int lbm_ensemble_remove(lbm_ensemble *ens, int32_t slot);
+int lbm_ensemble_count(const lbm_ensemble *ens, int32_t *slots, int32_t *resident, uint64_t *cells);
+/* The member context in a slot (owned by the ensemble). */
+int lbm_ensemble_cavity(lbm_ensemble *ens, int32_t slot, lbm_ctx **ctx);
+/* nsteps launches; a cavity still in the reference state first gets its
+ * prologue (its own step count then advances by nsteps + 1). */
+int lbm_ensemble_step(lbm_ensemble *ens, int64_t nsteps);
+int lbm_ensemble_step_timed(lbm_ensemble *ens, int64_t nsteps, double *elapsed_ms);
+/* nsteps launches, the last with every cavity's diagnostics folded in;
+ * out[slot] receives them (step = the cavity's completed step).  cap >= slots. */
+int lbm_ensemble_run_diag(lbm_ensemble *ens, int64_t nsteps, lbm_diag *out, int32_t cap);
+/* block_x (64, 128, 256) cells per tile, blocks_per_cu of the persistent
+ * grid, graph_pairs step pairs per graph replay (0: no graph). */
+int lbm_ensemble_set_tuning(lbm_ensemble *ens, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs);
+int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs);
+/* Times each (block_x, blocks_per_cu) candidate for `steps` steps on the live
+ * ensemble and keeps the fastest; with cache_path, a result stored for the
+ * same precision and cavity-size multiset is reused without timing. */
+int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_path, double *best_ms_per_step);
+
 /* Host copies of the reference state (the D2H copies of src/output.cu).
  * Fields: LX*LY*LZ doubles; populations: 27*LX*LY*LZ doubles. */
 int lbm_get_field(lbm_ctx *ctx, int32_t field, double *host);

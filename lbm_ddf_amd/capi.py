@@ -95,6 +95,21 @@ SIGNATURES = {
     "lbm_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "lbm_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _D]),
     "lbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, _D]),
+    "lbm_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_P)]),
+    "lbm_ensemble_destroy": (ctypes.c_int, [_P]),
+    "lbm_ensemble_add": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_int32)]),
+    "lbm_ensemble_remove": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "lbm_ensemble_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "lbm_ensemble_cavity": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "lbm_ensemble_step": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "lbm_ensemble_step_timed": (ctypes.c_int, [_P, ctypes.c_int64, _D]),
+    "lbm_ensemble_run_diag": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(Diag), ctypes.c_int32]),
+    "lbm_ensemble_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "lbm_ensemble_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.POINTER(ctypes.c_int32)]),
+    "lbm_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, _D]),
 }
 
 
@@ -301,3 +316,99 @@ class Cavity:
             "rho": self.get_field("rho"), "ux": self.get_field("ux"), "uy": self.get_field("uy"),
             "uz": self.get_field("uz"), "phi": self.get_field("phi"),
         }
+
+
+class _Member(Cavity):
+    """A cavity owned by an Ensemble: read through the single-cavity calls."""
+
+    def __init__(self, ens, slot, params):
+        self.lib = load()
+        self.params = params
+        self.shape = (int(params.lz), int(params.ly), int(params.lx))
+        self.n = int(params.lx) * int(params.ly) * int(params.lz)
+        self._ens = ens
+        self.slot = slot
+        h = _P()
+        _check(self.lib.lbm_ensemble_cavity(ens.h, slot, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):  # owned by the ensemble
+        self.h = None
+
+
+class Ensemble:
+    """Many cavities advanced by one launch per step (lbm_ensemble_*)."""
+
+    def __init__(self, precision: str = "fp64"):
+        self.lib = load()
+        self.precision = precision
+        h = _P()
+        _check(self.lib.lbm_ensemble_create(PREC[precision], ctypes.byref(h)))
+        self.h = h
+        self.params = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, lx, ly, lz, seed: int, amp: float = 1.0e-3, **kw) -> int:
+        p = make_params(lx, ly, lz, precision=self.precision, **kw)
+        slot = ctypes.c_int32()
+        _check(self.lib.lbm_ensemble_add(self.h, ctypes.byref(p), ctypes.c_uint64(seed), float(amp),
+                                         ctypes.byref(slot)))
+        self.params[slot.value] = p
+        return slot.value
+
+    def remove(self, slot: int):
+        _check(self.lib.lbm_ensemble_remove(self.h, int(slot)))
+        self.params.pop(slot, None)
+
+    def count(self) -> dict:
+        s, r, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
+        _check(self.lib.lbm_ensemble_count(self.h, ctypes.byref(s), ctypes.byref(r), ctypes.byref(n)))
+        return {"slots": s.value, "resident": r.value, "cells": int(n.value)}
+
+    def cavity(self, slot: int) -> Cavity:
+        return _Member(self, slot, self.params[slot])
+
+    def step(self, n: int):
+        _check(self.lib.lbm_ensemble_step(self.h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib.lbm_ensemble_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def run_diag(self, n: int) -> dict:
+        slots = self.count()["slots"]
+        arr = (Diag * max(1, slots))()
+        _check(self.lib.lbm_ensemble_run_diag(self.h, int(n), arr, max(1, slots)))
+        return {s: arr[s].as_dict() for s in self.params}
+
+    def set_tuning(self, block_x: int, blocks_per_cu: int, graph_pairs: int):
+        _check(self.lib.lbm_ensemble_set_tuning(self.h, int(block_x), int(blocks_per_cu), int(graph_pairs)))
+
+    @property
+    def tuning(self) -> tuple:
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.lbm_ensemble_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, steps: int = 8, cache_path: str | None = None) -> float:
+        ms = ctypes.c_double()
+        path = cache_path.encode() if cache_path else None
+        _check(self.lib.lbm_ensemble_autotune(self.h, int(steps), path, ctypes.byref(ms)))
+        return ms.value

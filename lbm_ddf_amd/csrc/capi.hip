@@ -37,7 +37,6 @@ int hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, #call);  \
     } while (0)
 
-enum Phase { PHASE_EMPTY = 0, PHASE_REF = 1, PHASE_FUSED = 2 };
 
 size_t elem_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 2; }
 size_t side_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 4; }
@@ -61,31 +60,9 @@ int validate(const lbm_params *p) {
 
 }  // namespace
 
-struct lbm_ctx {
-    lbm_params p{};
-    lbm_derived d{};
-    double beta = 0.0;
-    uint64_t n = 0;
-    uint64_t ps = 0;  // population plane stride (elements)
-    int prec = 0;
-    void *f[2] = {nullptr, nullptr};
-    void *g[2] = {nullptr, nullptr};
-    int cf = 0, cg = 0;
-    double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *phi = nullptr, *fy = nullptr;
-    void *lag = nullptr;
-    void *wall_rho = nullptr;
-    double *partials = nullptr;
-    lbm_diag *diag_dev = nullptr;
-    int64_t diag_cap = 0;
-    double *plane = nullptr;  // one population plane in FP64 (codec staging)
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int phase = PHASE_EMPTY;
-    int64_t nstep = 0;
-    int bx = 256;
-    int flags = 0;
-    int device = 0;
-};
+using lbm::PHASE_EMPTY;
+using lbm::PHASE_FUSED;
+using lbm::PHASE_REF;
 
 namespace {
 
@@ -265,12 +242,37 @@ int lbm_destroy(lbm_ctx *c) {
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBM_OK;
 }
 
-int lbm_create(const lbm_params *p, lbm_ctx **out) {
+int lbm_create(const lbm_params *p, lbm_ctx **out) { return lbm::ctx_create(p, nullptr, out); }
+
+}  // extern "C"
+
+namespace lbm {
+
+int ctx_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+StepArgs ctx_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, void *gb) {
+    return make_args(c, fa, ga, fb, gb);
+}
+
+int ctx_prologue(lbm_ctx *c) {
+    if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
+    if (c->phase != PHASE_REF) return LBM_OK;
+    if (int rc = run_pass(c, lbm::MODE_PROLOGUE, false)) return rc;
+    c->phase = PHASE_FUSED;
+    c->nstep += 1;
+    return LBM_OK;
+}
+
+int ctx_ensure_ref(lbm_ctx *c) { return ensure_ref(c); }
+
+// Creates a context; with stream != nullptr the context borrows that stream
+// (an ensemble's) instead of creating its own.
+int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
     int rc = validate(p);
     if (rc) return rc;
     if (!out) return fail(LBM_E_INVALID, "out is NULL");
@@ -287,7 +289,12 @@ int lbm_create(const lbm_params *p, lbm_ctx **out) {
         lbm_destroy(c);
         return hip_fail(e, "hipGetDevice");
     }
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    }
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
@@ -329,6 +336,10 @@ int lbm_create(const lbm_params *p, lbm_ctx **out) {
     *out = c;
     return LBM_OK;
 }
+
+}  // namespace lbm
+
+extern "C" {
 
 int lbm_init_arrays(lbm_ctx *c) {
     if (int rc = check_ctx(c)) return rc;

@@ -1,0 +1,485 @@
+// ensemble.hip -- many Rayleigh-Benard cavities of uneven size and physics
+// advanced by one persistent launch per step.
+//
+// Each cavity is an lbm_ctx (own buffers, fields and side values) that
+// borrows the ensemble's stream, so every reference-state operation (finalize,
+// host copies, reductions) is the single-cavity code.  The ensemble adds:
+//
+//   * a flat tile space: tile = BX consecutive cells of one cavity; cavity k
+//     owns tiles [tile0_k, tile0_k + ntiles_k).  A device descriptor table
+//     (EnsCavity) holds each cavity's step arguments for both buffer
+//     parities; the kernel finds a tile's cavity by binary search over tile0;
+//   * a persistent grid (blocks_per_cu x CUs blocks) that strides over the
+//     tiles, so the grid shape -- and a captured hipGraph -- does not depend
+//     on which cavities are resident;
+//   * hipGraph replay of step pairs (parity 0, 1): the table is rewritten
+//     between replays, never inside one;
+//   * per-cavity diagnostics folded into a step (per-tile partials, then one
+//     block per cavity sums them in a fixed order);
+//   * a launch-shape autotuner with an on-disk cache.
+//
+// The per-cell arithmetic is step_cell(), the same code as the single-cavity
+// kernel, so a cavity's state after k steps in the ensemble equals a
+// standalone run of k steps bit for bit.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ensemble_kernel.hpp"
+
+namespace lbm {
+
+hipError_t launch_ens_lp(int prec, const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx,
+                         int grid, hipStream_t s);
+
+hipError_t launch_ens_f64(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int grid,
+                          hipStream_t s) {
+    return launch_prec<PolicyF64>(cav, ctrl, parity, reduce, bx, grid, s);
+}
+
+namespace {
+
+// One block per resident cavity: fixed-order sum of its tile partials.
+__global__ __launch_bounds__(kFinishThreads) void ens_finish_kernel(const EnsCavity *__restrict__ cav,
+                                                                    lbm_diag *__restrict__ out) {
+    const EnsCavity &c = cav[blockIdx.x];
+    if (!c.active) return;
+    diag_finish_block(c.a[0].partials, c.ntiles, c.d, out + c.slot);
+}
+
+hipError_t launch_ens(int prec, const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx,
+                      int grid, hipStream_t s) {
+    switch (prec) {
+        case LBM_PREC_FP64: return launch_ens_f64(cav, ctrl, parity, reduce, bx, grid, s);
+        case LBM_PREC_FP16:
+        case LBM_PREC_BF16: return launch_ens_lp(prec, cav, ctrl, parity, reduce, bx, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+}  // namespace lbm
+
+struct lbm_ensemble {
+    int prec = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<lbm_ctx *> slots;  // nullptr = free slot
+    // device tables
+    lbm::EnsCavity *cav_dev = nullptr;
+    uint32_t *ctrl_dev = nullptr;
+    lbm_diag *diag_dev = nullptr;
+    size_t cav_cap = 0;
+    // pinned staging for the tables
+    lbm::EnsCavity *cav_host = nullptr;
+    uint32_t *ctrl_host = nullptr;
+    size_t host_cap = 0;
+    uint32_t ncav = 0;  // resident cavities in the table
+    // tuning
+    int bx = 128;
+    int blocks_per_cu = 8;
+    int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
+    int num_cus = 256;
+    // graph of graph_pairs (parity 0, parity 1) launches
+    hipGraphExec_t graph = nullptr;
+    bool tables_dirty = true;
+    uint64_t tile_cap = 0;
+};
+
+namespace {
+
+using lbm::ctx_fail;
+
+#define ENS_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return ctx_fail(LBM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int grid_of(const lbm_ensemble *e) { return std::max(1, e->blocks_per_cu * e->num_cus); }
+
+void drop_graph(lbm_ensemble *e) {
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+}
+
+int ensure_tables(lbm_ensemble *e) {
+    const size_t need = std::max<size_t>(1, e->slots.size());
+    if (need > e->cav_cap) {
+        ENS_HIP(hipStreamSynchronize(e->stream));
+        if (e->cav_dev) (void)hipFree(e->cav_dev);
+        if (e->diag_dev) (void)hipFree(e->diag_dev);
+        e->cav_dev = nullptr;
+        e->diag_dev = nullptr;
+        size_t cap = std::max<size_t>(need, 2 * e->cav_cap);
+        if (hipMalloc(&e->cav_dev, sizeof(lbm::EnsCavity) * cap) != hipSuccess ||
+            hipMalloc(&e->diag_dev, sizeof(lbm_diag) * cap) != hipSuccess)
+            return ctx_fail(LBM_E_NOMEM, "ensemble tables");
+        e->cav_cap = cap;
+        drop_graph(e);  // the graph captured the old table pointer
+    }
+    if (need > e->host_cap) {
+        if (e->cav_host) (void)hipHostFree(e->cav_host);
+        e->cav_host = nullptr;
+        size_t cap = e->cav_cap;
+        if (hipHostMalloc(reinterpret_cast<void **>(&e->cav_host), sizeof(lbm::EnsCavity) * cap) != hipSuccess)
+            return ctx_fail(LBM_E_NOMEM, "ensemble host staging");
+        e->host_cap = cap;
+    }
+    if (!e->ctrl_dev) {
+        if (hipMalloc(&e->ctrl_dev, 4 * sizeof(uint32_t)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&e->ctrl_host), 4 * sizeof(uint32_t)) != hipSuccess)
+            return ctx_fail(LBM_E_NOMEM, "ensemble control words");
+    }
+    return LBM_OK;
+}
+
+// Writes the descriptor table for the resident cavities (all in FUSED phase)
+// with parity 0 = their current buffers.  Uploaded on the ensemble stream.
+int upload_tables(lbm_ensemble *e) {
+    if (int rc = ensure_tables(e)) return rc;
+    // The staging buffer may still feed an in-flight copy.
+    ENS_HIP(hipStreamSynchronize(e->stream));
+    uint32_t ncav = 0, tile = 0;
+    uint64_t tiles = 0;
+    for (size_t s = 0; s < e->slots.size(); ++s) {
+        lbm_ctx *c = e->slots[s];
+        if (!c) continue;
+        lbm::EnsCavity &d = e->cav_host[ncav];
+        std::memset(&d, 0, sizeof(d));
+        d.a[0] = lbm::ctx_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
+        d.a[1] = lbm::ctx_args(c, c->f[1 - c->cf], c->g[1 - c->cg], c->f[c->cf], c->g[c->cg]);
+        d.d.lx = c->p.lx;
+        d.d.ly = c->p.ly;
+        d.d.lz = c->p.lz;
+        d.d.diff = c->d.diff;
+        d.d.dT = c->p.t_hot - c->p.t_cold;
+        d.d.step = 0;
+        d.ntiles = static_cast<uint32_t>((c->n + e->bx - 1) / e->bx);
+        d.tile0 = tile;
+        d.active = c->phase == lbm::PHASE_FUSED ? 1 : 0;
+        d.slot = static_cast<int32_t>(s);
+        tile += d.ntiles;
+        tiles += d.ntiles;
+        ++ncav;
+    }
+    if (tiles >= (1ull << 32)) return ctx_fail(LBM_E_INVALID, "too many tiles");
+    e->ncav = ncav;
+    e->ctrl_host[0] = ncav;
+    e->ctrl_host[1] = static_cast<uint32_t>(tiles);
+    e->ctrl_host[2] = e->ctrl_host[3] = 0;
+    if (ncav) ENS_HIP(hipMemcpyAsync(e->cav_dev, e->cav_host, sizeof(lbm::EnsCavity) * ncav, hipMemcpyHostToDevice, e->stream));
+    ENS_HIP(hipMemcpyAsync(e->ctrl_dev, e->ctrl_host, 4 * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
+    e->tables_dirty = false;
+    return LBM_OK;
+}
+
+// Every resident cavity must be in FUSED phase before a fused launch.
+int prologue_all(lbm_ensemble *e) {
+    bool changed = false;
+    for (lbm_ctx *c : e->slots) {
+        if (!c) continue;
+        if (c->phase == lbm::PHASE_EMPTY) return ctx_fail(LBM_E_STATE, "ensemble cavity has no state");
+        if (c->phase == lbm::PHASE_REF) {
+            if (int rc = lbm::ctx_prologue(c)) return rc;
+            changed = true;
+        }
+    }
+    if (changed) e->tables_dirty = true;
+    return LBM_OK;
+}
+
+int launch_one(lbm_ensemble *e, int parity, bool reduce) {
+    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, parity, reduce, e->bx, grid_of(e), e->stream));
+    return LBM_OK;
+}
+
+int build_graph(lbm_ensemble *e) {
+    drop_graph(e);
+    hipGraph_t g = nullptr;
+    ENS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < e->graph_pairs; ++k) {
+        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 0, false, e->bx, grid_of(e), e->stream);
+        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 1, false, e->bx, grid_of(e), e->stream);
+        if (e0 != hipSuccess || e1 != hipSuccess) {
+            (void)hipStreamEndCapture(e->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return ctx_fail(LBM_E_HIP, "kernel launch during graph capture");
+        }
+    }
+    ENS_HIP(hipStreamEndCapture(e->stream, &g));
+    hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) {
+        e->graph = nullptr;
+        return ctx_fail(LBM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+    }
+    return LBM_OK;
+}
+
+// nsteps fused launches over all resident cavities (after their prologues).
+int run_fused(lbm_ensemble *e, int64_t nsteps, bool reduce_last) {
+    if (int rc = prologue_all(e)) return rc;
+    if (e->tables_dirty)
+        if (int rc = upload_tables(e)) return rc;
+    if (e->ncav == 0 || nsteps <= 0) return LBM_OK;
+    int64_t done = 0;
+    const int64_t per_graph = 2 * static_cast<int64_t>(e->graph_pairs);
+    const int64_t plain = reduce_last ? nsteps - 1 : nsteps;
+    if (e->graph_pairs > 0 && plain >= per_graph) {
+        if (!e->graph)
+            if (int rc = build_graph(e)) return rc;
+        while (plain - done >= per_graph) {
+            ENS_HIP(hipGraphLaunch(e->graph, e->stream));
+            done += per_graph;
+        }
+    }
+    while (done < plain) {
+        if (int rc = launch_one(e, static_cast<int>(done & 1), false)) return rc;
+        ++done;
+    }
+    if (reduce_last) {
+        if (int rc = launch_one(e, static_cast<int>(done & 1), true)) return rc;
+        ++done;
+    }
+    // bookkeeping: every resident FUSED cavity advanced by nsteps
+    for (lbm_ctx *c : e->slots) {
+        if (!c || c->phase != lbm::PHASE_FUSED) continue;
+        if (nsteps & 1) {
+            c->cf = 1 - c->cf;
+            c->cg = 1 - c->cg;
+        }
+        c->nstep += nsteps;
+    }
+    if (nsteps & 1) e->tables_dirty = true;
+    return LBM_OK;
+}
+
+lbm_ctx *cavity(lbm_ensemble *e, int32_t slot) {
+    if (!e || slot < 0 || static_cast<size_t>(slot) >= e->slots.size()) return nullptr;
+    return e->slots[slot];
+}
+
+std::string tuning_key(const lbm_ensemble *e) {
+    // precision + sorted multiset of (lx, ly, lz), hashed
+    std::vector<uint64_t> dims;
+    for (const lbm_ctx *c : e->slots)
+        if (c) dims.push_back((static_cast<uint64_t>(c->p.lx) << 40) | (static_cast<uint64_t>(c->p.ly) << 20) | c->p.lz);
+    std::sort(dims.begin(), dims.end());
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t v : dims) {
+        h ^= v;
+        h *= 1099511628211ull;
+    }
+    std::ostringstream os;
+    os << "p" << e->prec << "_n" << dims.size() << "_" << std::hex << h;
+    return os.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_ensemble_create(int32_t precision, lbm_ensemble **out) {
+    if (!out) return ctx_fail(LBM_E_INVALID, "out is NULL");
+    if (precision < LBM_PREC_FP64 || precision > LBM_PREC_BF16) return ctx_fail(LBM_E_INVALID, "unknown precision");
+    *out = nullptr;
+    lbm_ensemble *e = new lbm_ensemble();
+    e->prec = precision;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        e->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        delete e;
+        return ctx_fail(LBM_E_HIP, "ensemble stream/events");
+    }
+    *out = e;
+    return LBM_OK;
+}
+
+int lbm_ensemble_destroy(lbm_ensemble *e) {
+    if (!e) return LBM_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (lbm_ctx *c : e->slots)
+        if (c) lbm_destroy(c);
+    if (e->cav_dev) (void)hipFree(e->cav_dev);
+    if (e->diag_dev) (void)hipFree(e->diag_dev);
+    if (e->ctrl_dev) (void)hipFree(e->ctrl_dev);
+    if (e->cav_host) (void)hipHostFree(e->cav_host);
+    if (e->ctrl_host) (void)hipHostFree(e->ctrl_host);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return LBM_OK;
+}
+
+int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double amp, int32_t *slot) {
+    if (!e || !p || !slot) return ctx_fail(LBM_E_INVALID, "NULL argument");
+    if (p->precision != e->prec) return ctx_fail(LBM_E_INVALID, "cavity precision differs from the ensemble's");
+    lbm_ctx *c = nullptr;
+    if (int rc = lbm::ctx_create(p, e->stream, &c)) return rc;
+    if (int rc = lbm_init_conductive(c, seed, amp)) {
+        lbm_destroy(c);
+        return rc;
+    }
+    size_t s = 0;
+    while (s < e->slots.size() && e->slots[s]) ++s;
+    if (s == e->slots.size()) e->slots.push_back(c);
+    else e->slots[s] = c;
+    *slot = static_cast<int32_t>(s);
+    e->tables_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_ensemble_remove(lbm_ensemble *e, int32_t slot) {
+    lbm_ctx *c = cavity(e, slot);
+    if (!c) return ctx_fail(LBM_E_INVALID, "no cavity in that slot");
+    (void)hipStreamSynchronize(e->stream);
+    lbm_destroy(c);
+    e->slots[slot] = nullptr;
+    e->tables_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_ensemble_count(const lbm_ensemble *e, int32_t *slots, int32_t *resident, uint64_t *cells) {
+    if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
+    int32_t r = 0;
+    uint64_t n = 0;
+    for (const lbm_ctx *c : e->slots)
+        if (c) {
+            ++r;
+            n += c->n;
+        }
+    if (slots) *slots = static_cast<int32_t>(e->slots.size());
+    if (resident) *resident = r;
+    if (cells) *cells = n;
+    return LBM_OK;
+}
+
+int lbm_ensemble_step(lbm_ensemble *e, int64_t nsteps) {
+    if (!e || nsteps < 0) return ctx_fail(LBM_E_INVALID, "bad ensemble/nsteps");
+    return run_fused(e, nsteps, false);
+}
+
+int lbm_ensemble_step_timed(lbm_ensemble *e, int64_t nsteps, double *elapsed_ms) {
+    if (!e || nsteps < 0 || !elapsed_ms) return ctx_fail(LBM_E_INVALID, "bad arguments");
+    if (int rc = prologue_all(e)) return rc;
+    if (e->tables_dirty)
+        if (int rc = upload_tables(e)) return rc;
+    ENS_HIP(hipEventRecord(e->ev0, e->stream));
+    if (int rc = run_fused(e, nsteps, false)) return rc;
+    ENS_HIP(hipEventRecord(e->ev1, e->stream));
+    ENS_HIP(hipEventSynchronize(e->ev1));
+    float ms = 0.0f;
+    ENS_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    *elapsed_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_ensemble_run_diag(lbm_ensemble *e, int64_t nsteps, lbm_diag *out, int32_t cap) {
+    if (!e || nsteps < 1 || !out) return ctx_fail(LBM_E_INVALID, "bad arguments");
+    if (cap < static_cast<int32_t>(e->slots.size())) return ctx_fail(LBM_E_INVALID, "out must hold one record per slot");
+    // the reduced step of each cavity is the one the last launch completes
+    if (int rc = prologue_all(e)) return rc;
+    std::vector<int64_t> step(e->slots.size(), -1);
+    for (size_t s = 0; s < e->slots.size(); ++s)
+        if (e->slots[s]) step[s] = e->slots[s]->nstep + nsteps - 1;
+    if (int rc = run_fused(e, nsteps, true)) return rc;
+    if (e->ncav) {
+        hipLaunchKernelGGL(lbm::ens_finish_kernel, dim3(e->ncav), dim3(lbm::kFinishThreads), 0, e->stream, e->cav_dev,
+                           e->diag_dev);
+        ENS_HIP(hipGetLastError());
+        ENS_HIP(hipMemcpyAsync(out, e->diag_dev, sizeof(lbm_diag) * e->slots.size(), hipMemcpyDeviceToHost, e->stream));
+    }
+    ENS_HIP(hipStreamSynchronize(e->stream));
+    for (size_t s = 0; s < e->slots.size(); ++s) {
+        if (!e->slots[s]) std::memset(&out[s], 0, sizeof(lbm_diag));
+        out[s].step = step[s];
+    }
+    return LBM_OK;
+}
+
+int lbm_ensemble_cavity(lbm_ensemble *e, int32_t slot, lbm_ctx **ctx) {
+    lbm_ctx *c = cavity(e, slot);
+    if (!c || !ctx) return ctx_fail(LBM_E_INVALID, "no cavity in that slot");
+    *ctx = c;
+    return LBM_OK;
+}
+
+int lbm_ensemble_set_tuning(lbm_ensemble *e, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs) {
+    if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
+    if (block_x != 64 && block_x != 128 && block_x != 256) return ctx_fail(LBM_E_INVALID, "block_x must be 64, 128 or 256");
+    if (blocks_per_cu < 1 || blocks_per_cu > 64 || graph_pairs < 0 || graph_pairs > 256)
+        return ctx_fail(LBM_E_INVALID, "blocks_per_cu in 1..64, graph_pairs in 0..256");
+    if (block_x != e->bx) e->tables_dirty = true;  // tile count per cavity changes
+    if (block_x != e->bx || blocks_per_cu != e->blocks_per_cu || graph_pairs != e->graph_pairs) drop_graph(e);
+    e->bx = block_x;
+    e->blocks_per_cu = blocks_per_cu;
+    e->graph_pairs = graph_pairs;
+    return LBM_OK;
+}
+
+int lbm_ensemble_get_tuning(const lbm_ensemble *e, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs) {
+    if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
+    if (block_x) *block_x = e->bx;
+    if (blocks_per_cu) *blocks_per_cu = e->blocks_per_cu;
+    if (graph_pairs) *graph_pairs = e->graph_pairs;
+    return LBM_OK;
+}
+
+int lbm_ensemble_autotune(lbm_ensemble *e, int32_t steps, const char *cache_path, double *best_ms_per_step) {
+    if (!e || steps < 2) return ctx_fail(LBM_E_INVALID, "steps must be >= 2");
+    const std::string key = tuning_key(e);
+    if (cache_path && *cache_path) {
+        std::ifstream in(cache_path);
+        std::string k;
+        int bx, bpc, gp;
+        double ms;
+        while (in >> k >> bx >> bpc >> gp >> ms) {
+            if (k == key) {
+                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp)) return rc;
+                if (best_ms_per_step) *best_ms_per_step = ms;
+                return LBM_OK;
+            }
+        }
+    }
+    const int blocks[] = {64, 128, 256};
+    const int bpcs[] = {2, 4, 8, 16};
+    int best_bx = e->bx, best_bpc = e->blocks_per_cu;
+    double best = 1e300;
+    const int gp = std::max(1, steps / 2);
+    for (int bx : blocks)
+        for (int bpc : bpcs) {
+            if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp)) return rc;
+            if (int rc = lbm_ensemble_step(e, 2 * gp)) return rc;  // builds the graph, warms caches
+            double ms = 0.0;
+            if (int rc = lbm_ensemble_step_timed(e, 2 * gp, &ms)) return rc;
+            if (ms < best) {
+                best = ms;
+                best_bx = bx;
+                best_bpc = bpc;
+            }
+        }
+    if (int rc = lbm_ensemble_set_tuning(e, best_bx, best_bpc, gp)) return rc;
+    best /= 2 * gp;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    if (cache_path && *cache_path) {
+        std::ofstream o(cache_path, std::ios::app);
+        o.precision(17);
+        o << key << " " << best_bx << " " << best_bpc << " " << gp << " " << best << "\n";
+    }
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -100,4 +100,46 @@ inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
     return ntx * (int64_t)ly * (int64_t)lz;
 }
 
+enum Phase { PHASE_EMPTY = 0, PHASE_REF = 1, PHASE_FUSED = 2 };
+
+}  // namespace lbm
+
+// One cavity's device state (opaque to C callers).  Buffers f[2], g[2] hold
+// 27 planes each (stride ps); in REF phase f[cf], g[cg] are the reference's
+// post-streaming populations, in FUSED phase its post-collision ones.
+struct lbm_ctx {
+    lbm_params p{};
+    lbm_derived d{};
+    double beta = 0.0;
+    uint64_t n = 0;
+    uint64_t ps = 0;  // population plane stride (elements)
+    int prec = 0;
+    void *f[2] = {nullptr, nullptr};
+    void *g[2] = {nullptr, nullptr};
+    int cf = 0, cg = 0;
+    double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *phi = nullptr, *fy = nullptr;
+    void *lag = nullptr;
+    void *wall_rho = nullptr;
+    double *partials = nullptr;
+    lbm_diag *diag_dev = nullptr;
+    int64_t diag_cap = 0;
+    double *plane = nullptr;  // one population plane in FP64 (codec staging)
+    hipStream_t stream = nullptr;
+    bool own_stream = true;  // false when an ensemble lends its stream
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int phase = lbm::PHASE_EMPTY;
+    int64_t nstep = 0;
+    int bx = 256;
+    int flags = 0;
+    int device = 0;
+};
+
+
+namespace lbm {
+// Helpers shared by capi.hip and ensemble.hip (defined in capi.hip).
+int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out);
+StepArgs ctx_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, void *gb);
+int ctx_prologue(lbm_ctx *c);    // REF -> FUSED (collides the next step)
+int ctx_ensure_ref(lbm_ctx *c);  // FUSED -> REF (finalize)
+int ctx_fail(int code, const std::string &msg);
 }  // namespace lbm

@@ -229,7 +229,7 @@ __global__ __launch_bounds__(BX) void reduce_fields_kernel(StepArgs a) {
     } else {
         diag_terms(q, false, iy, a.ly, 0.0, 0.0, 0.0, 0.0, 0.0);
     }
-    block_reduce_diag<BX>(q, a.partials);
+    block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(blockIdx.x) * NQ);
 }
 
 // Sums the per-block partials in a fixed order and applies the formulas of
@@ -237,54 +237,9 @@ __global__ __launch_bounds__(BX) void reduce_fields_kernel(StepArgs a) {
 // gradients are linear in phi, so the plane sums give the same plane means:
 //   <dT/dy>_bottom = (-3 S0 + 4 S1 - S2) / 2 / (LX*LZ)      (:211, :236-253)
 //   <dT/dy>_top    = ( 3 T0 - 4 T1 + T2) / 2 / (LX*LZ)      (:216)
-constexpr int kFinishThreads = 256;
 __global__ __launch_bounds__(kFinishThreads) void diag_finish_kernel(const double *partials, int64_t nblocks,
                                                                     DiagArgs d, lbm_diag *dst) {
-    __shared__ double sh[kFinishThreads][NQ + 1];
-    const int t = threadIdx.x;
-    double acc[NQ];
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
-    for (int64_t j = t; j < nblocks; j += kFinishThreads) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            const double v = partials[j * NQ + k];
-            acc[k] = (k == Q_MAXUSQ) ? fmax(acc[k], v) : acc[k] + v;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) sh[t][k] = acc[k];
-    __syncthreads();
-    for (int s = kFinishThreads / 2; s >= 1; s >>= 1) {
-        if (t < s) {
-#pragma unroll
-            for (int k = 0; k < NQ; ++k)
-                sh[t][k] = (k == Q_MAXUSQ) ? fmax(sh[t][k], sh[t + s][k]) : sh[t][k] + sh[t + s][k];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        const double ncell = static_cast<double>(d.lx) * d.ly * d.lz;
-        const double nplane = static_cast<double>(d.lx) * d.lz;
-        const double *S = sh[0];
-        lbm_diag r;
-        r.step = d.step;
-        const double uyT_avg = S[Q_UYPHI] / ncell;
-        r.nu_bulk = 1.0 + uyT_avg * d.ly / (d.diff * d.dT);
-        const double dtdy_bottom = ((-3.0 * S[Q_B0] + 4.0 * S[Q_B1] - S[Q_B2]) / 2.0) / nplane;
-        const double dtdy_top = ((3.0 * S[Q_T0] - 4.0 * S[Q_T1] + S[Q_T2]) / 2.0) / nplane;
-        r.nu_top = -dtdy_top * d.ly / d.dT;
-        r.nu_bottom = -dtdy_bottom * d.ly / d.dT;
-        r.umean = S[Q_UX] / ncell;
-        r.vmean = S[Q_UY] / ncell;
-        r.wmean = S[Q_UZ] / ncell;
-        r.tmean = S[Q_PHI] / ncell;
-        r.kinetic_energy = 0.5 * S[Q_USQ] / ncell;
-        r.mass = S[Q_RHO];
-        r.max_speed = sqrt(S[Q_MAXUSQ]);
-        r.nonfinite = S[Q_NONFINITE];
-        *dst = r;
-    }
+    diag_finish_block(partials, nblocks, d, dst);
 }
 
 // output_profile, src/output.cu:336-358: one block per y-plane.

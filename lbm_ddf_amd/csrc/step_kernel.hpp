@@ -106,7 +106,7 @@ struct PolicyBF16 {
 // ---------------------------------------------------------------------------
 
 template <int BX>
-__device__ __forceinline__ void block_reduce_diag(double (&q)[NQ], double *partials) {
+__device__ __forceinline__ void block_reduce_diag(double (&q)[NQ], double *out) {
     static_assert(BX % 64 == 0, "block must be whole waves");
     constexpr int NW = BX / 64;
     __shared__ double red[NW][NQ];
@@ -131,8 +131,9 @@ __device__ __forceinline__ void block_reduce_diag(double (&q)[NQ], double *parti
         const int k = threadIdx.x;
         double v = red[0][k];
         for (int w = 1; w < NW; ++w) v = (k == Q_MAXUSQ) ? fmax(v, red[w][k]) : v + red[w][k];
-        partials[static_cast<uint64_t>(blockIdx.x) * NQ + k] = v;
+        out[k] = v;
     }
+    __syncthreads();  // red[][] may be rewritten by the caller's next tile
 }
 
 // Per-cell diagnostic terms of a completed step.
@@ -159,6 +160,59 @@ __device__ __forceinline__ void diag_terms(double (&q)[NQ], bool active, int iy,
     q[Q_MAXUSQ] = finite ? usq : 0.0;
     q[Q_NONFINITE] = finite ? 0.0 : 1.0;
 }
+
+// Sums per-block partials in a fixed order (one block of kFinishThreads) and
+// applies the formulas of src/output.cu:228-256 (Nu) and :31-34 (means).
+constexpr int kFinishThreads = 256;
+__device__ __forceinline__ void diag_finish_block(const double *partials, int64_t nblocks, const DiagArgs &d,
+                                                  lbm_diag *dst) {
+    __shared__ double sh[kFinishThreads][NQ + 1];
+    const int t = threadIdx.x;
+    double acc[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
+    for (int64_t j = t; j < nblocks; j += kFinishThreads) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const double v = partials[j * NQ + k];
+            acc[k] = (k == Q_MAXUSQ) ? fmax(acc[k], v) : acc[k] + v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) sh[t][k] = acc[k];
+    __syncthreads();
+    for (int s = kFinishThreads / 2; s >= 1; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+                sh[t][k] = (k == Q_MAXUSQ) ? fmax(sh[t][k], sh[t + s][k]) : sh[t][k] + sh[t + s][k];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double ncell = static_cast<double>(d.lx) * d.ly * d.lz;
+        const double nplane = static_cast<double>(d.lx) * d.lz;
+        const double *S = sh[0];
+        lbm_diag r;
+        r.step = d.step;
+        const double uyT_avg = S[Q_UYPHI] / ncell;
+        r.nu_bulk = 1.0 + uyT_avg * d.ly / (d.diff * d.dT);
+        const double dtdy_bottom = ((-3.0 * S[Q_B0] + 4.0 * S[Q_B1] - S[Q_B2]) / 2.0) / nplane;
+        const double dtdy_top = ((3.0 * S[Q_T0] - 4.0 * S[Q_T1] + S[Q_T2]) / 2.0) / nplane;
+        r.nu_top = -dtdy_top * d.ly / d.dT;
+        r.nu_bottom = -dtdy_bottom * d.ly / d.dT;
+        r.umean = S[Q_UX] / ncell;
+        r.vmean = S[Q_UY] / ncell;
+        r.wmean = S[Q_UZ] / ncell;
+        r.tmean = S[Q_PHI] / ncell;
+        r.kinetic_energy = 0.5 * S[Q_USQ] / ncell;
+        r.mass = S[Q_RHO];
+        r.max_speed = sqrt(S[Q_MAXUSQ]);
+        r.nonfinite = S[Q_NONFINITE];
+        *dst = r;
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // Collision of one cell, f and g (src/collision.cu:22-32 and :74-81).
@@ -273,152 +327,156 @@ __device__ __forceinline__ void st(S *p, S v) {
 // The step kernel.
 // ---------------------------------------------------------------------------
 
-template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
-__global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
+// One lattice cell of a step pass: gather (or plain load in the prologue),
+// moments, forcing, collision, stores, side values, and the cell's terms of
+// the diagnostics.  Shared by the single-cavity and the ensemble kernels.
+template <class P, int MODE, bool REDUCE, int FLAGS>
+__device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
     using S = typename P::S;
     using T = typename P::T;
     using L = typename P::L;
-
     const int lx = a.lx, ly = a.ly, lz = a.lz;
+    const uint64_t n = a.ps;  // population plane stride
+    const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
+    const S *__restrict__ fa = static_cast<const S *>(a.fa);
+    const S *__restrict__ ga = static_cast<const S *>(a.ga);
+    S *__restrict__ fb = static_cast<S *>(a.fb);
+    S *__restrict__ gb = static_cast<S *>(a.gb);
+    const T t0 = static_cast<T>(a.t0);
+
+    T f[Q], g[Q];
+    T rho, ux, uy, uz, phi;
+    const bool wall_lo = iy == 0, wall_hi = iy == ly - 1;
+    const uint64_t wall_idx = (static_cast<uint64_t>(wall_hi ? 1 : 0) * lz + iz) * lx + ix;
+    L *__restrict__ wall_rho = static_cast<L *>(a.wall_rho);
+    L *__restrict__ lag = static_cast<L *>(a.lag);
+
+    if constexpr (MODE == MODE_PROLOGUE) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            f[i] = P::dec_f(ld<NTL>(&fa[i * n + idx]));
+            g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), t0);
+        });
+        rho = static_cast<T>(a.rho[idx]);
+        ux = static_cast<T>(a.ux[idx]);
+        uy = static_cast<T>(a.uy[idx]);
+        uz = static_cast<T>(a.uz[idx]);
+        phi = static_cast<T>(a.phi[idx]);
+    } else {
+        // ---- gather (pull) with the wall rules -------------------------
+        const bool xlo = ix > 0, xhi = ix < lx - 1;
+        const bool ylo = iy > 0, yhi = iy < ly - 1;
+        const bool zlo = iz > 0, zhi = iz < lz - 1;
+        const int sy = lx, sz = lx * ly;
+        T rw = T(0);
+        if (wall_lo || wall_hi) rw = static_cast<T>(wall_rho[wall_idx]);
+        const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
+            if constexpr (i == 0) {
+                f[0] = P::dec_f(ld<NTL>(&fa[idx]));
+                g[0] = P::template dec_g<0>(ld<NTL>(&ga[idx]), t0);
+            } else {
+                const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
+                const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
+                const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+                if (inx && iny && inz) {
+                    const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
+                    f[i] = P::dec_f(ld<NTL && cx == 0>(&fa[i * n + src]));
+                    g[i] = P::template dec_g<i>(ld<NTL && cx == 0>(&ga[i * n + src]), t0);
+                } else {
+                    // Half-way bounce-back from the cell's own opposite
+                    // population (f at every wall; g at the x and z walls).
+                    f[i] = P::dec_f(fa[o * n + idx]);
+                    T gv = P::template dec_g<o>(ga[o * n + idx], t0);
+                    if constexpr (cy != 0) {
+                        // Isothermal anti-bounce-back at the y walls; the
+                        // reference applies the z walls after the y walls,
+                        // so an out-of-domain z source keeps bounce-back.
+                        if (!iny && inz) {
+                            constexpr T w2 = static_cast<T>(2.0 * weight(i));
+                            gv = -gv + w2 * tw * (T(1) + rw);
+                        }
+                    }
+                    g[i] = gv;
+                }
+            }
+        });
+        // ---- moments, summed in direction order (src/macrovar.cu:17-27)
+        T jx = T(0), jy = T(0), jz = T(0);
+        rho = T(0);
+        phi = T(0);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            rho = rho + f[i];
+            if constexpr (CX[i] > 0) jx = jx + f[i];
+            if constexpr (CX[i] < 0) jx = jx - f[i];
+            if constexpr (CY[i] > 0) jy = jy + f[i];
+            if constexpr (CY[i] < 0) jy = jy - f[i];
+            if constexpr (CZ[i] > 0) jz = jz + f[i];
+            if constexpr (CZ[i] < 0) jz = jz - f[i];
+            phi = phi + g[i];
+        });
+        const T phi_prev = static_cast<T>(lag[idx]);
+        const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
+        ux = jx;
+        uy = jy + T(0.5) * fprev;
+        uz = jz;
+        if constexpr (MODE == MODE_FINALIZE) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
+                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
+            });
+            a.rho[idx] = static_cast<double>(rho);
+            a.ux[idx] = static_cast<double>(ux);
+            a.uy[idx] = static_cast<double>(uy);
+            a.uz[idx] = static_cast<double>(uz);
+            a.phi[idx] = static_cast<double>(phi);
+            a.fy[idx] = static_cast<double>(fprev);
+        }
+    }
+
+    if constexpr (MODE != MODE_FINALIZE) {
+        // ---- forcing (src/macrovar.cu:45) and collision -----------------
+        const T F = static_cast<T>(a.grav_beta) * (phi - t0);
+        collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
+            st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
+        });
+        lag[idx] = static_cast<L>(phi);
+        if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);
+    }
+
+    if constexpr (REDUCE && MODE != MODE_PROLOGUE) {
+        diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux),
+                   static_cast<double>(uy), static_cast<double>(uz), static_cast<double>(phi));
+    }
+}
+
+template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
+__global__ __launch_bounds__(BX) void step_kernel(const StepArgs a) {
+    const int lx = a.lx, ly = a.ly;
     const uint32_t b = (FLAGS & kXcdSwizzle) ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t row = b / static_cast<uint32_t>(a.ntx);
     const int bxi = static_cast<int>(b - row * static_cast<uint32_t>(a.ntx));
     const int iy = static_cast<int>(row % static_cast<uint32_t>(ly));
     const int iz = static_cast<int>(row / static_cast<uint32_t>(ly));
     const int ix = bxi * BX + static_cast<int>(threadIdx.x);
-    const bool active = ix < lx;
 
     double q[NQ];
     if constexpr (REDUCE) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) q[k] = 0.0;
     }
-
-    if (active) {
-        const uint64_t n = a.ps;  // population plane stride
-        const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
-        const S *__restrict__ fa = static_cast<const S *>(a.fa);
-        const S *__restrict__ ga = static_cast<const S *>(a.ga);
-        S *__restrict__ fb = static_cast<S *>(a.fb);
-        S *__restrict__ gb = static_cast<S *>(a.gb);
-        const T t0 = static_cast<T>(a.t0);
-
-        T f[Q], g[Q];
-        T rho, ux, uy, uz, phi;
-        const bool wall_lo = iy == 0, wall_hi = iy == ly - 1;
-        const uint64_t wall_idx = (static_cast<uint64_t>(wall_hi ? 1 : 0) * lz + iz) * lx + ix;
-        L *__restrict__ wall_rho = static_cast<L *>(a.wall_rho);
-        L *__restrict__ lag = static_cast<L *>(a.lag);
-
-        if constexpr (MODE == MODE_PROLOGUE) {
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                f[i] = P::dec_f(ld<NTL>(&fa[i * n + idx]));
-                g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), t0);
-            });
-            rho = static_cast<T>(a.rho[idx]);
-            ux = static_cast<T>(a.ux[idx]);
-            uy = static_cast<T>(a.uy[idx]);
-            uz = static_cast<T>(a.uz[idx]);
-            phi = static_cast<T>(a.phi[idx]);
-        } else {
-            // ---- gather (pull) with the wall rules -------------------------
-            const bool xlo = ix > 0, xhi = ix < lx - 1;
-            const bool ylo = iy > 0, yhi = iy < ly - 1;
-            const bool zlo = iz > 0, zhi = iz < lz - 1;
-            const int sy = lx, sz = lx * ly;
-            T rw = T(0);
-            if (wall_lo || wall_hi) rw = static_cast<T>(wall_rho[wall_idx]);
-            const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
-                if constexpr (i == 0) {
-                    f[0] = P::dec_f(ld<NTL>(&fa[idx]));
-                    g[0] = P::template dec_g<0>(ld<NTL>(&ga[idx]), t0);
-                } else {
-                    const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
-                    const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
-                    const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
-                    if (inx && iny && inz) {
-                        const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
-                        f[i] = P::dec_f(ld<NTL && cx == 0>(&fa[i * n + src]));
-                        g[i] = P::template dec_g<i>(ld<NTL && cx == 0>(&ga[i * n + src]), t0);
-                    } else {
-                        // Half-way bounce-back from the cell's own opposite
-                        // population (f at every wall; g at the x and z walls).
-                        f[i] = P::dec_f(fa[o * n + idx]);
-                        T gv = P::template dec_g<o>(ga[o * n + idx], t0);
-                        if constexpr (cy != 0) {
-                            // Isothermal anti-bounce-back at the y walls; the
-                            // reference applies the z walls after the y walls,
-                            // so an out-of-domain z source keeps bounce-back.
-                            if (!iny && inz) {
-                                constexpr T w2 = static_cast<T>(2.0 * weight(i));
-                                gv = -gv + w2 * tw * (T(1) + rw);
-                            }
-                        }
-                        g[i] = gv;
-                    }
-                }
-            });
-            // ---- moments, summed in direction order (src/macrovar.cu:17-27)
-            T jx = T(0), jy = T(0), jz = T(0);
-            rho = T(0);
-            phi = T(0);
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                rho = rho + f[i];
-                if constexpr (CX[i] > 0) jx = jx + f[i];
-                if constexpr (CX[i] < 0) jx = jx - f[i];
-                if constexpr (CY[i] > 0) jy = jy + f[i];
-                if constexpr (CY[i] < 0) jy = jy - f[i];
-                if constexpr (CZ[i] > 0) jz = jz + f[i];
-                if constexpr (CZ[i] < 0) jz = jz - f[i];
-                phi = phi + g[i];
-            });
-            const T phi_prev = static_cast<T>(lag[idx]);
-            const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
-            ux = jx;
-            uy = jy + T(0.5) * fprev;
-            uz = jz;
-            if constexpr (MODE == MODE_FINALIZE) {
-                static_for<Q>([&](auto I) {
-                    constexpr int i = I;
-                    st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-                    st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
-                });
-                a.rho[idx] = static_cast<double>(rho);
-                a.ux[idx] = static_cast<double>(ux);
-                a.uy[idx] = static_cast<double>(uy);
-                a.uz[idx] = static_cast<double>(uz);
-                a.phi[idx] = static_cast<double>(phi);
-                a.fy[idx] = static_cast<double>(fprev);
-            }
-        }
-
-        if constexpr (MODE != MODE_FINALIZE) {
-            // ---- forcing (src/macrovar.cu:45) and collision -----------------
-            const T F = static_cast<T>(a.grav_beta) * (phi - t0);
-            collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
-            });
-            lag[idx] = static_cast<L>(phi);
-            if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);
-        }
-
-        if constexpr (REDUCE && MODE != MODE_PROLOGUE) {
-            diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux),
-                       static_cast<double>(uy), static_cast<double>(uz), static_cast<double>(phi));
-        }
-    }
-
-    if constexpr (REDUCE && MODE != MODE_PROLOGUE) block_reduce_diag<BX>(q, a.partials);
+    if (ix < lx) step_cell<P, MODE, REDUCE, FLAGS>(a, ix, iy, iz, q);
+    if constexpr (REDUCE && MODE != MODE_PROLOGUE)
+        block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(blockIdx.x) * NQ);
 }
 
 // Variant flags used by the plain fused pass; the other passes use FLAGS=0.

@@ -1,0 +1,135 @@
+"""Ensemble (many cavities, one launch per step) against standalone cavities.
+
+Each cavity's arithmetic in the ensemble kernel is the single-cavity
+step_cell(), so after k of its own steps a member must equal a standalone
+run of k steps bit for bit -- alone and in the mix, across hipGraph replays
+with eager work between them, and across joins and removals.
+"""
+import numpy as np
+import pytest
+
+from lbm_ddf_amd import Cavity, Ensemble
+
+pytestmark = pytest.mark.gpu
+
+CAVS = [
+    dict(shape=(12, 10, 8), rayleigh=1e5, prandtl=0.71, seed=11),
+    dict(shape=(16, 9, 11), rayleigh=3e5, prandtl=1.0, seed=12),
+    dict(shape=(7, 12, 5), rayleigh=1e6, prandtl=4.4, seed=13),
+    dict(shape=(20, 16, 14), rayleigh=5e4, prandtl=0.71, seed=14),
+    dict(shape=(9, 8, 9), rayleigh=2e6, prandtl=1.0, seed=15),
+]
+WALLS = dict(t_hot=0.5, t_cold=-0.5)
+AMP = 1.0e-2
+
+
+def standalone(cav, nsteps, precision="fp64"):
+    c = Cavity(*cav["shape"], rayleigh=cav["rayleigh"], prandtl=cav["prandtl"], precision=precision, **WALLS)
+    c.init_conductive(cav["seed"], AMP)
+    c.step(nsteps)
+    return c
+
+
+def build(precision="fp64", cavs=CAVS):
+    e = Ensemble(precision)
+    slots = [e.add(*c["shape"], seed=c["seed"], amp=AMP, rayleigh=c["rayleigh"], prandtl=c["prandtl"], **WALLS)
+             for c in cavs]
+    return e, slots
+
+
+def assert_member_equals(e, slot, cav, precision="fp64"):
+    m = e.cavity(slot)
+    k = m.step_count
+    ref = standalone(cav, k, precision)
+    a, b = m.state(), ref.state()
+    for key in a:
+        assert np.array_equal(a[key], b[key]), f"slot {slot} {key} differs after {k} steps"
+    return k
+
+
+@pytest.mark.parametrize("graph_pairs", [0, 4])
+def test_members_equal_standalone_runs(graph_pairs):
+    e, slots = build()
+    e.set_tuning(128, 4, graph_pairs)
+    e.step(37)
+    for s, cav in zip(slots, CAVS):
+        # prologue (collides step 1) + 37 launches
+        assert assert_member_equals(e, s, cav) == 38
+    e.close()
+
+
+def test_graph_replays_with_eager_work_between():
+    """Replays of the captured graph interleaved with host reads (which
+    finalize members) and direct launches: every replay stays exact."""
+    e, slots = build()
+    e.set_tuning(64, 2, 3)  # 6 steps per replay
+    total = 0
+    for n in (12, 7, 18):
+        e.step(n)
+        total += n + 1  # members were read (reference state): prologue + n launches
+        ks = [assert_member_equals(e, s, c) for s, c in zip(slots, CAVS)]
+        assert ks == [total] * len(slots)
+    e.close()
+
+
+def test_run_diag_matches_standalone_reduce():
+    e, slots = build()
+    e.step(10)
+    diags = e.run_diag(5)
+    for s, cav in zip(slots, CAVS):
+        d = diags[s]
+        ref = standalone(cav, d["step"])
+        r = ref.reduce()
+        for k in ("nu_bulk", "nu_top", "nu_bottom", "umean", "vmean", "wmean", "tmean", "kinetic_energy",
+                  "mass", "max_speed"):
+            assert np.isclose(d[k], r[k], rtol=1e-11, atol=1e-14), (s, k, d[k], r[k])
+        assert d["nonfinite"] == 0
+    e.close()
+
+
+def test_join_and_leave_between_calls():
+    e, slots = build(cavs=CAVS[:3])
+    e.step(9)
+    s4 = e.add(*CAVS[3]["shape"], seed=CAVS[3]["seed"], amp=AMP, rayleigh=CAVS[3]["rayleigh"],
+               prandtl=CAVS[3]["prandtl"], **WALLS)
+    e.step(8)
+    e.remove(slots[1])
+    s5 = e.add(*CAVS[4]["shape"], seed=CAVS[4]["seed"], amp=AMP, rayleigh=CAVS[4]["rayleigh"],
+               prandtl=CAVS[4]["prandtl"], **WALLS)
+    assert s5 == slots[1]  # the freed slot is reused
+    e.step(11)
+    assert e.count()["resident"] == 4
+    assert assert_member_equals(e, slots[0], CAVS[0]) == 29  # prologue + 9 + 8 + 11
+    assert assert_member_equals(e, slots[2], CAVS[2]) == 29
+    assert assert_member_equals(e, s4, CAVS[3]) == 20  # joined: prologue + 8 + 11
+    assert assert_member_equals(e, s5, CAVS[4]) == 12  # prologue + 11
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_reduced_precision_members_equal_standalone(precision):
+    e, slots = build(precision)
+    e.step(21)
+    for s, cav in zip(slots, CAVS):
+        assert_member_equals(e, s, cav, precision)
+    e.close()
+
+
+def test_autotune_and_cache(tmp_path):
+    e, slots = build()
+    cache = str(tmp_path / "tune.txt")
+    ms = e.autotune(4, cache)
+    assert ms > 0
+    t1 = e.tuning
+    lines = open(cache).read().split("\n")
+    assert len([x for x in lines if x.strip()]) == 1
+    e2, _ = build()
+    ms2 = e2.autotune(4, cache)  # served from the cache: same choice, no new line
+    assert e2.tuning == t1
+    assert ms2 == pytest.approx(ms, rel=1e-12)
+    assert len([x for x in open(cache).read().split("\n") if x.strip()]) == 1
+    # results are unaffected by the tuning
+    for s, cav in zip(slots, CAVS):
+        assert_member_equals(e, s, cav)
+    e.close()
+    e2.close()
