@@ -73,6 +73,81 @@ def cpu_baseline(size: int, ra: float, budget_s: float):
     }
 
 
+def ensemble_cavities(count: int = 64, draw_seed: int = 4):
+    """Config of the 64-cavity ensemble: lateral side from {32,48,64,96,128},
+    height from {32,64,128}, Ra log-uniform in [1e4, 1e7], Pr from
+    {0.71, 1.0, 4.4}, cavity k seeded 1000+k; draws from numpy's PCG64 with
+    seed `draw_seed`."""
+    import numpy as np
+
+    rng = np.random.default_rng(draw_seed)
+    cavs = []
+    for k in range(count):
+        lat = int(rng.choice([32, 48, 64, 96, 128]))
+        h = int(rng.choice([32, 64, 128]))
+        ra = float(10 ** rng.uniform(4.0, 7.0))
+        pr = float(rng.choice([0.71, 1.0, 4.4]))
+        cavs.append(dict(lx=lat, ly=h, lz=lat, rayleigh=ra, prandtl=pr, seed=1000 + k))
+    return cavs
+
+
+def run_ensemble_workload(args) -> dict:
+    """64 cavities in one persistent launch per step (hipGraph replay,
+    autotuned launch shape) against the same cavities run one after another
+    as single-cavity contexts."""
+    from lbm_ddf_amd import Cavity, Ensemble
+
+    cavs = ensemble_cavities()
+    cells = sum(c["lx"] * c["ly"] * c["lz"] for c in cavs)
+    out = {"cavities": len(cavs), "cells": cells, "precisions": {}}
+    for prec in args.ensemble_precisions.split(","):
+        e = Ensemble(prec)
+        for c in cavs:
+            e.add(c["lx"], c["ly"], c["lz"], seed=c["seed"], amp=1.0e-3, rayleigh=c["rayleigh"],
+                  prandtl=c["prandtl"], t_hot=0.5, t_cold=-0.5)
+        e.step(2)
+        t = time.perf_counter()
+        e.autotune(8, str(ROOT / "profiles" / "ensemble_tuning.txt") if args.tuning_cache else None)
+        bx, bpc, gp, fl = e.tuning
+        e.set_tuning(bx, bpc, 16, fl)
+        log(f"ensemble {prec}: autotune {time.perf_counter() - t:.1f} s -> block {bx}, {bpc} blocks/CU, "
+            f"flags {fl}")
+        e.step(32)
+        e.count()
+        t0 = time.perf_counter()
+        ms = e.step_timed(args.ensemble_steps)
+        wall = time.perf_counter() - t0
+        agg = cells * args.ensemble_steps / wall / 1e6
+        diags = e.run_diag(1)
+        bad = sum(1 for d in diags.values() if d["nonfinite"] != 0)
+        e.close()
+        log(f"ensemble {prec}: {args.ensemble_steps} steps in {wall:.2f} s -> {agg:.0f} MLUPS aggregate")
+        # sequential single-cavity runs, bounded sample of steps per cavity
+        seq_steps = args.sequential_steps
+        tot = 0.0
+        for c in cavs:
+            cv = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], t_hot=0.5,
+                        t_cold=-0.5, precision=prec)
+            cv.init_conductive(c["seed"], 1.0e-3)
+            cv.step(4)
+            cv.synchronize()
+            t1 = time.perf_counter()
+            cv.step(seq_steps)
+            cv.synchronize()
+            tot += time.perf_counter() - t1
+            cv.close()
+        seq = cells * seq_steps / tot / 1e6
+        log(f"sequential {prec}: {seq:.0f} MLUPS ({seq_steps} steps per cavity, {tot:.2f} s)")
+        out["precisions"][prec] = {
+            "aggregate_mlups": round(agg, 1), "ms_per_step": round(wall / args.ensemble_steps * 1e3, 4),
+            "event_ms_per_step": round(ms / args.ensemble_steps, 4), "sequential_mlups": round(seq, 1),
+            "speedup_vs_sequential": round(agg / seq, 2),
+            "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "graph_steps": 32},
+            "nonfinite_cavities": bad,
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +161,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+                    help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--ensemble-precisions", default="fp64,fp16")
+    ap.add_argument("--sequential-steps", type=int, default=300)
+    ap.add_argument("--tuning-cache", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
                          "the HBM-counter calibration run")
@@ -103,6 +184,22 @@ def main() -> int:
         print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
               "note (DESIGN.md), not an implemented path. Run with --gpus 1.", file=sys.stderr)
         return 2
+
+    if args.workload == "ensemble":
+        res = run_ensemble_workload(args)
+        main_prec = args.ensemble_precisions.split(",")[0]
+        r = res["precisions"][main_prec]
+        print(json.dumps({
+            "metric": "aggregate MLUPS, 64-cavity ensemble in one launch per step",
+            "value": r["aggregate_mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.ensemble_steps,
+            "warmup": 34, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if main_prec == "fp64" else f"{main_prec} storage, f32 arithmetic",
+            "data": "synthetic: 64 cavities, sizes/Ra/Pr drawn with seed 4, cavity k seeded 1000+k",
+            "config": {"workload": "64 Rayleigh-Benard cavities, lateral {32..128}, height {32,64,128}, "
+                                   "Ra log-U[1e4,1e7], Pr {0.71,1,4.4}", "cells": res["cells"]},
+            "ensemble": res,
+        }), flush=True)
+        return 0
 
     from lbm_ddf_amd import Cavity, bytes_required, make_params
 

@@ -202,13 +202,75 @@ int lbm_ensemble_step_timed(lbm_ensemble *ens, int64_t nsteps, double *elapsed_m
  * out[slot] receives them (step = the cavity's completed step).  cap >= slots. */
 int lbm_ensemble_run_diag(lbm_ensemble *ens, int64_t nsteps, lbm_diag *out, int32_t cap);
 /* block_x (64, 128, 256) cells per tile, blocks_per_cu of the persistent
- * grid, graph_pairs step pairs per graph replay (0: no graph). */
-int lbm_ensemble_set_tuning(lbm_ensemble *ens, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs);
-int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs);
-/* Times each (block_x, blocks_per_cu) candidate for `steps` steps on the live
+ * grid, graph_pairs step pairs per graph replay (0: no graph), flags 0, 3
+ * (non-temporal hints), 4 (XCD-contiguous tile order) or 7 (both). */
+int lbm_ensemble_set_tuning(lbm_ensemble *ens, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs,
+                            int32_t flags);
+int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs,
+                            int32_t *flags);
+/* Times each (block_x, blocks_per_cu, flags) candidate for `steps` steps on the live
  * ensemble and keeps the fastest; with cache_path, a result stored for the
  * same precision and cavity-size multiset is reused without timing. */
 int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_path, double *best_ms_per_step);
+
+/* ---- Serving: continuous batching of arriving solves ----------------------
+ * No reference counterpart.  Each request is one cavity (conductive start);
+ * lbm_serve runs the requests in real time on one GPU: between step chunks
+ * it admits every arrived request that fits the HBM budget (first fit in
+ * arrival order), advances all resident cavities together (ensemble, one
+ * launch per step), checks each request's stopping rule and retires finished
+ * ones.  Stopping rule: every `window` of its own steps a request samples
+ * Nu_bulk; its running Nu is the mean of the samples; it is converged when the
+ * running Nu moves by less than nu_tol between checks (>= 3 samples), else it
+ * stops at max_steps.  A request's result does not depend on the mix. */
+#define LBM_REQ_CONVERGED 0  /* stopping rule met */
+#define LBM_REQ_BUDGET 1     /* max_steps reached */
+#define LBM_REQ_REJECTED 2   /* alone larger than the HBM budget */
+#define LBM_REQ_DIVERGED 3   /* non-finite moments: retired, others continue */
+#define LBM_REQ_UNFINISHED 4 /* admitted, time limit reached first */
+#define LBM_REQ_PENDING 5    /* never admitted before the time limit */
+
+typedef struct lbm_request {
+    int32_t lx, ly, lz;
+    int32_t reserved;
+    double rayleigh, prandtl;
+    double t_hot, t_cold;
+    uint64_t seed;
+    double amp;        /* perturbation amplitude of the conductive start */
+    double arrival_s;  /* seconds after the start of lbm_serve */
+    int64_t max_steps;
+    double nu_tol;
+    int64_t window;    /* multiple of lbm_serve_options.chunk_steps */
+} lbm_request;
+
+typedef struct lbm_result {
+    int32_t status; /* LBM_REQ_* */
+    int32_t reserved;
+    double arrival_s, admit_s, finish_s;
+    int64_t steps;
+    double nu_bulk, nu_top, nu_bottom, kinetic_energy; /* at the last check */
+    double nu_running;
+} lbm_result;
+
+typedef struct lbm_serve_options {
+    int32_t precision;
+    int32_t max_resident; /* 0: no limit besides memory */
+    int64_t chunk_steps;  /* steps between admissions/checks */
+    uint64_t hbm_budget;  /* bytes for resident cavities */
+    double time_limit_s;
+    int32_t block_x, blocks_per_cu, flags; /* ensemble launch shape; block_x 0: default */
+} lbm_serve_options;
+
+typedef struct lbm_serve_stats {
+    double wall_s, busy_s;
+    double cell_updates, mlups;
+    int32_t completed, rejected, diverged, unfinished, pending, peak_resident;
+    uint64_t peak_bytes;
+    double p50_s, p99_s; /* time to result (finish - arrival) of completed requests */
+} lbm_serve_stats;
+
+int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, lbm_result *results,
+              lbm_serve_stats *stats);
 
 /* Host copies of the reference state (the D2H copies of src/output.cu).
  * Fields: LX*LY*LZ doubles; populations: 27*LX*LY*LZ doubles. */

@@ -5,8 +5,8 @@ The compute path is the HIP library built from ``lbm_ddf_amd/csrc`` (see
 binding; importing the package does not load the library, so that a
 CPU-only environment can import it to check the build.
 """
-from .capi import (Cavity, Ensemble, LbmError, LIB_PATH, PREC, FIELDS, POPS, bytes_required, derive, load,
+from .capi import (Cavity, Ensemble, LbmError, serve, LIB_PATH, PREC, FIELDS, POPS, bytes_required, derive, load,
                    make_params)
 
-__all__ = ["Cavity", "Ensemble", "LbmError", "LIB_PATH", "PREC", "FIELDS", "POPS", "bytes_required", "derive",
+__all__ = ["Cavity", "Ensemble", "LbmError", "serve", "LIB_PATH", "PREC", "FIELDS", "POPS", "bytes_required", "derive",
            "load", "make_params"]

@@ -106,9 +106,9 @@ SIGNATURES = {
     "lbm_ensemble_step": (ctypes.c_int, [_P, ctypes.c_int64]),
     "lbm_ensemble_step_timed": (ctypes.c_int, [_P, ctypes.c_int64, _D]),
     "lbm_ensemble_run_diag": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(Diag), ctypes.c_int32]),
-    "lbm_ensemble_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "lbm_ensemble_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "lbm_ensemble_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
-                                               ctypes.POINTER(ctypes.c_int32)]),
+                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "lbm_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, _D]),
 }
 
@@ -398,17 +398,84 @@ class Ensemble:
         _check(self.lib.lbm_ensemble_run_diag(self.h, int(n), arr, max(1, slots)))
         return {s: arr[s].as_dict() for s in self.params}
 
-    def set_tuning(self, block_x: int, blocks_per_cu: int, graph_pairs: int):
-        _check(self.lib.lbm_ensemble_set_tuning(self.h, int(block_x), int(blocks_per_cu), int(graph_pairs)))
+    def set_tuning(self, block_x: int, blocks_per_cu: int, graph_pairs: int, flags: int = 7):
+        _check(self.lib.lbm_ensemble_set_tuning(self.h, int(block_x), int(blocks_per_cu), int(graph_pairs),
+                                                int(flags)))
 
     @property
     def tuning(self) -> tuple:
-        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(self.lib.lbm_ensemble_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """(block_x, blocks_per_cu, graph_pairs, flags)."""
+        a, b, c, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.lbm_ensemble_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def autotune(self, steps: int = 8, cache_path: str | None = None) -> float:
         ms = ctypes.c_double()
         path = cache_path.encode() if cache_path else None
         _check(self.lib.lbm_ensemble_autotune(self.h, int(steps), path, ctypes.byref(ms)))
         return ms.value
+
+
+# ---------------------------------------------------------------- serving
+class Request(ctypes.Structure):
+    _fields_ = [("lx", ctypes.c_int32), ("ly", ctypes.c_int32), ("lz", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("rayleigh", ctypes.c_double), ("prandtl", ctypes.c_double),
+                ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("amp", ctypes.c_double), ("arrival_s", ctypes.c_double), ("max_steps", ctypes.c_int64),
+                ("nu_tol", ctypes.c_double), ("window", ctypes.c_int64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("reserved", ctypes.c_int32), ("arrival_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("nu_bulk", ctypes.c_double), ("nu_top", ctypes.c_double), ("nu_bottom", ctypes.c_double),
+                ("kinetic_energy", ctypes.c_double), ("nu_running", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d["status"] = REQ_STATUS[self.status]
+        return d
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("chunk_steps", ctypes.c_int64),
+                ("hbm_budget", ctypes.c_uint64), ("time_limit_s", ctypes.c_double), ("block_x", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double), ("cell_updates", ctypes.c_double),
+                ("mlups", ctypes.c_double), ("completed", ctypes.c_int32), ("rejected", ctypes.c_int32),
+                ("diverged", ctypes.c_int32), ("unfinished", ctypes.c_int32), ("pending", ctypes.c_int32),
+                ("peak_resident", ctypes.c_int32), ("peak_bytes", ctypes.c_uint64), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_STATUS = {0: "converged", 1: "budget", 2: "rejected", 3: "diverged", 4: "unfinished", 5: "pending"}
+SIGNATURES["lbm_serve"] = (ctypes.c_int, [ctypes.POINTER(Request), ctypes.c_int32, ctypes.POINTER(ServeOptions),
+                                          ctypes.POINTER(Result), ctypes.POINTER(ServeStats)])
+
+
+def serve(requests: list, precision: str = "fp64", chunk_steps: int = 100, hbm_budget: float = 250e9,
+          time_limit_s: float = 60.0, max_resident: int = 0, block_x: int = 0, blocks_per_cu: int = 8,
+          flags: int = 7):
+    """Run lbm_serve over a list of request dicts (keys of Request; defaults:
+    walls +-0.5, amp 1e-3).  Returns (results list of dicts, stats dict)."""
+    lib = load()
+    n = len(requests)
+    arr = (Request * max(1, n))()
+    for i, r in enumerate(requests):
+        q = dict(t_hot=0.5, t_cold=-0.5, amp=1.0e-3, prandtl=0.71, reserved=0)
+        q.update(r)
+        for k, _ in Request._fields_:
+            setattr(arr[i], k, q[k])
+    opt = ServeOptions(PREC[precision], int(max_resident), int(chunk_steps), int(hbm_budget), float(time_limit_s),
+                       int(block_x), int(blocks_per_cu), int(flags))
+    res = (Result * max(1, n))()
+    st = ServeStats()
+    _check(lib.lbm_serve(arr, n, ctypes.byref(opt), res, ctypes.byref(st)))
+    return [res[i].as_dict() for i in range(n)], st.as_dict()

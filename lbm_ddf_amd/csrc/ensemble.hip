@@ -35,11 +35,11 @@
 namespace lbm {
 
 hipError_t launch_ens_lp(int prec, const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx,
-                         int grid, hipStream_t s);
+                         int flags, int grid, hipStream_t s);
 
-hipError_t launch_ens_f64(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int grid,
-                          hipStream_t s) {
-    return launch_prec<PolicyF64>(cav, ctrl, parity, reduce, bx, grid, s);
+hipError_t launch_ens_f64(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int flags,
+                          int grid, hipStream_t s) {
+    return launch_prec<PolicyF64>(cav, ctrl, parity, reduce, bx, flags, grid, s);
 }
 
 namespace {
@@ -53,11 +53,11 @@ __global__ __launch_bounds__(kFinishThreads) void ens_finish_kernel(const EnsCav
 }
 
 hipError_t launch_ens(int prec, const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx,
-                      int grid, hipStream_t s) {
+                      int flags, int grid, hipStream_t s) {
     switch (prec) {
-        case LBM_PREC_FP64: return launch_ens_f64(cav, ctrl, parity, reduce, bx, grid, s);
+        case LBM_PREC_FP64: return launch_ens_f64(cav, ctrl, parity, reduce, bx, flags, grid, s);
         case LBM_PREC_FP16:
-        case LBM_PREC_BF16: return launch_ens_lp(prec, cav, ctrl, parity, reduce, bx, grid, s);
+        case LBM_PREC_BF16: return launch_ens_lp(prec, cav, ctrl, parity, reduce, bx, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -84,6 +84,7 @@ struct lbm_ensemble {
     int bx = 128;
     int blocks_per_cu = 8;
     int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
+    int flags = 7;        // launch variant of the plain step (0, 3, 4, 7)
     int num_cus = 256;
     // graph of graph_pairs (parity 0, parity 1) launches
     hipGraphExec_t graph = nullptr;
@@ -197,7 +198,7 @@ int prologue_all(lbm_ensemble *e) {
 }
 
 int launch_one(lbm_ensemble *e, int parity, bool reduce) {
-    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, parity, reduce, e->bx, grid_of(e), e->stream));
+    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, parity, reduce, e->bx, e->flags, grid_of(e), e->stream));
     return LBM_OK;
 }
 
@@ -206,8 +207,8 @@ int build_graph(lbm_ensemble *e) {
     hipGraph_t g = nullptr;
     ENS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < e->graph_pairs; ++k) {
-        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 0, false, e->bx, grid_of(e), e->stream);
-        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 1, false, e->bx, grid_of(e), e->stream);
+        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 0, false, e->bx, e->flags, grid_of(e), e->stream);
+        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, 1, false, e->bx, e->flags, grid_of(e), e->stream);
         if (e0 != hipSuccess || e1 != hipSuccess) {
             (void)hipStreamEndCapture(e->stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -417,67 +418,76 @@ int lbm_ensemble_cavity(lbm_ensemble *e, int32_t slot, lbm_ctx **ctx) {
     return LBM_OK;
 }
 
-int lbm_ensemble_set_tuning(lbm_ensemble *e, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs) {
+int lbm_ensemble_set_tuning(lbm_ensemble *e, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs,
+                            int32_t flags) {
     if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
     if (block_x != 64 && block_x != 128 && block_x != 256) return ctx_fail(LBM_E_INVALID, "block_x must be 64, 128 or 256");
     if (blocks_per_cu < 1 || blocks_per_cu > 64 || graph_pairs < 0 || graph_pairs > 256)
         return ctx_fail(LBM_E_INVALID, "blocks_per_cu in 1..64, graph_pairs in 0..256");
+    if (flags != 0 && flags != 3 && flags != 4 && flags != 7) return ctx_fail(LBM_E_INVALID, "flags must be 0, 3, 4 or 7");
     if (block_x != e->bx) e->tables_dirty = true;  // tile count per cavity changes
-    if (block_x != e->bx || blocks_per_cu != e->blocks_per_cu || graph_pairs != e->graph_pairs) drop_graph(e);
+    if (block_x != e->bx || blocks_per_cu != e->blocks_per_cu || graph_pairs != e->graph_pairs || flags != e->flags)
+        drop_graph(e);
     e->bx = block_x;
     e->blocks_per_cu = blocks_per_cu;
     e->graph_pairs = graph_pairs;
+    e->flags = flags;
     return LBM_OK;
 }
 
-int lbm_ensemble_get_tuning(const lbm_ensemble *e, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs) {
+int lbm_ensemble_get_tuning(const lbm_ensemble *e, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs,
+                            int32_t *flags) {
     if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
     if (block_x) *block_x = e->bx;
     if (blocks_per_cu) *blocks_per_cu = e->blocks_per_cu;
     if (graph_pairs) *graph_pairs = e->graph_pairs;
+    if (flags) *flags = e->flags;
     return LBM_OK;
 }
 
 int lbm_ensemble_autotune(lbm_ensemble *e, int32_t steps, const char *cache_path, double *best_ms_per_step) {
     if (!e || steps < 2) return ctx_fail(LBM_E_INVALID, "steps must be >= 2");
     const std::string key = tuning_key(e);
+    const int gp = std::max(1, steps / 2);
     if (cache_path && *cache_path) {
         std::ifstream in(cache_path);
         std::string k;
-        int bx, bpc, gp;
+        int bx, bpc, fl;
         double ms;
-        while (in >> k >> bx >> bpc >> gp >> ms) {
+        while (in >> k >> bx >> bpc >> fl >> ms) {
             if (k == key) {
-                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp)) return rc;
+                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp, fl)) return rc;
                 if (best_ms_per_step) *best_ms_per_step = ms;
                 return LBM_OK;
             }
         }
     }
     const int blocks[] = {64, 128, 256};
-    const int bpcs[] = {2, 4, 8, 16};
-    int best_bx = e->bx, best_bpc = e->blocks_per_cu;
+    const int bpcs[] = {4, 8, 16};
+    const int flagset[] = {0, 3, 4, 7};
+    int best_bx = e->bx, best_bpc = e->blocks_per_cu, best_fl = e->flags;
     double best = 1e300;
-    const int gp = std::max(1, steps / 2);
     for (int bx : blocks)
-        for (int bpc : bpcs) {
-            if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp)) return rc;
-            if (int rc = lbm_ensemble_step(e, 2 * gp)) return rc;  // builds the graph, warms caches
-            double ms = 0.0;
-            if (int rc = lbm_ensemble_step_timed(e, 2 * gp, &ms)) return rc;
-            if (ms < best) {
-                best = ms;
-                best_bx = bx;
-                best_bpc = bpc;
+        for (int bpc : bpcs)
+            for (int fl : flagset) {
+                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp, fl)) return rc;
+                if (int rc = lbm_ensemble_step(e, 2 * gp)) return rc;  // builds the graph, warms caches
+                double ms = 0.0;
+                if (int rc = lbm_ensemble_step_timed(e, 2 * gp, &ms)) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_bx = bx;
+                    best_bpc = bpc;
+                    best_fl = fl;
+                }
             }
-        }
-    if (int rc = lbm_ensemble_set_tuning(e, best_bx, best_bpc, gp)) return rc;
+    if (int rc = lbm_ensemble_set_tuning(e, best_bx, best_bpc, gp, best_fl)) return rc;
     best /= 2 * gp;
     if (best_ms_per_step) *best_ms_per_step = best;
     if (cache_path && *cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o.precision(17);
-        o << key << " " << best_bx << " " << best_bpc << " " << gp << " " << best << "\n";
+        o << key << " " << best_bx << " " << best_bpc << " " << best_fl << " " << best << "\n";
     }
     return LBM_OK;
 }

@@ -15,11 +15,30 @@ struct EnsCavity {
     int32_t slot;
 };
 
-template <class P, int MODE, bool REDUCE, int BX>
+template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
 __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restrict__ cav,
                                                       const uint32_t *__restrict__ ctrl, int parity) {
     const uint32_t ncav = ctrl[0], ntiles = ctrl[1];
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // Tile order.  Plain: block b takes tiles b, b+G, ...  XCD-contiguous:
+    // the blocks of XCD x (b % 8 == x) share the x-th eighth of the tile
+    // space, so neighbouring tiles -- which share the cache lines straddled by
+    // x-shifted loads -- run on one XCD's L2.
+    uint32_t t, t_end, t_step;
+    if constexpr ((FLAGS & kXcdSwizzle) != 0) {
+        constexpr uint32_t kXcds = 8;
+        const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+        const uint32_t gx = gridDim.x / kXcds + (x < gridDim.x % kXcds ? 1u : 0u);
+        const uint32_t q = ntiles / kXcds, r = ntiles % kXcds;
+        const uint32_t start = x * q + (x < r ? x : r);
+        t = start + k;
+        t_end = start + q + (x < r ? 1u : 0u);
+        t_step = gx;
+    } else {
+        t = blockIdx.x;
+        t_end = ntiles;
+        t_step = gridDim.x;
+    }
+    for (; t < t_end; t += t_step) {
         uint32_t lo = 0, hi = ncav;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -42,29 +61,44 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
             const int ix = static_cast<int>(cell - r * lx);
             const int iy = static_cast<int>(r % ly);
             const int iz = static_cast<int>(r / ly);
-            step_cell<P, MODE, REDUCE, 0>(a, ix, iy, iz, q);
+            step_cell<P, MODE, REDUCE, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, q);
         }
         if constexpr (REDUCE) block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(lt) * NQ);
     }
 }
 
-template <class P, int BX>
-hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int grid,
-                     hipStream_t s) {
-    if (reduce)
-        hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX>), dim3(grid), dim3(BX), 0, s, cav, ctrl, parity);
-    else
-        hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX>), dim3(grid), dim3(BX), 0, s, cav, ctrl, parity);
+template <class P, int BX, int FL>
+hipError_t launch_fl(const EnsCavity *cav, const uint32_t *ctrl, int parity, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX, FL>), dim3(grid), dim3(BX), 0, s, cav, ctrl, parity);
     return hipGetLastError();
 }
 
+// Flags of the plain launch: 0, 3 (non-temporal), 4 (XCD order), 7 (both).
+// The launch that folds the diagnostics uses the XCD order only.
+template <class P, int BX>
+hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int flags, int grid,
+                     hipStream_t s) {
+    if (reduce) {
+        hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX, kXcdSwizzle>), dim3(grid), dim3(BX), 0, s, cav,
+                           ctrl, parity);
+        return hipGetLastError();
+    }
+    switch (flags) {
+        case 0: return launch_fl<P, BX, 0>(cav, ctrl, parity, grid, s);
+        case 3: return launch_fl<P, BX, 3>(cav, ctrl, parity, grid, s);
+        case 4: return launch_fl<P, BX, 4>(cav, ctrl, parity, grid, s);
+        case 7: return launch_fl<P, BX, 7>(cav, ctrl, parity, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <class P>
-hipError_t launch_prec(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int grid,
-                       hipStream_t s) {
+hipError_t launch_prec(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int flags,
+                       int grid, hipStream_t s) {
     switch (bx) {
-        case 64: return launch_bx<P, 64>(cav, ctrl, parity, reduce, grid, s);
-        case 128: return launch_bx<P, 128>(cav, ctrl, parity, reduce, grid, s);
-        case 256: return launch_bx<P, 256>(cav, ctrl, parity, reduce, grid, s);
+        case 64: return launch_bx<P, 64>(cav, ctrl, parity, reduce, flags, grid, s);
+        case 128: return launch_bx<P, 128>(cav, ctrl, parity, reduce, flags, grid, s);
+        case 256: return launch_bx<P, 256>(cav, ctrl, parity, reduce, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -50,7 +50,7 @@ def assert_member_equals(e, slot, cav, precision="fp64"):
 @pytest.mark.parametrize("graph_pairs", [0, 4])
 def test_members_equal_standalone_runs(graph_pairs):
     e, slots = build()
-    e.set_tuning(128, 4, graph_pairs)
+    e.set_tuning(128, 4, graph_pairs, 7 if graph_pairs else 0)
     e.step(37)
     for s, cav in zip(slots, CAVS):
         # prologue (collides step 1) + 37 launches
@@ -62,7 +62,7 @@ def test_graph_replays_with_eager_work_between():
     """Replays of the captured graph interleaved with host reads (which
     finalize members) and direct launches: every replay stays exact."""
     e, slots = build()
-    e.set_tuning(64, 2, 3)  # 6 steps per replay
+    e.set_tuning(64, 2, 3, 4)  # 6 steps per replay
     total = 0
     for n in (12, 7, 18):
         e.step(n)

@@ -1,0 +1,82 @@
+"""Continuous-batching request loop (lbm_serve) on the GPU.
+
+A request's result must not depend on what else is resident: its cavity's
+arithmetic is the single-cavity code and its check points depend only on its
+own step count, so alone and in a mix the step count, Nu and status are
+identical (FP64: bit for bit).
+"""
+import numpy as np
+import pytest
+
+from lbm_ddf_amd import bytes_required, make_params, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def req(lx, ly, lz, ra, seed, arrival=0.0, max_steps=600, nu_tol=1e-12, window=100, **kw):
+    d = dict(lx=lx, ly=ly, lz=lz, rayleigh=ra, seed=seed, arrival_s=arrival, max_steps=max_steps, nu_tol=nu_tol,
+             window=window)
+    d.update(kw)
+    return d
+
+
+A = req(20, 16, 12, 1e5, 1, amp=1e-2)
+B = req(16, 24, 16, 3e5, 2, amp=1e-2)
+C = req(24, 12, 20, 1e6, 3, amp=1e-2, max_steps=900)
+D = req(12, 12, 12, 5e4, 4, amp=1e-2, max_steps=300)
+
+
+def same(r1, r2):
+    for k in ("status", "steps", "nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+        assert r1[k] == r2[k], (k, r1[k], r2[k])
+    assert (np.isnan(r1["nu_running"]) and np.isnan(r2["nu_running"])) or r1["nu_running"] == r2["nu_running"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp16"])
+def test_result_same_alone_and_in_mix(precision):
+    alone, _ = serve([A], precision=precision, chunk_steps=50)
+    assert alone[0]["status"] == "budget" and alone[0]["steps"] == 600
+    mix = [dict(B, arrival_s=0.0), dict(A, arrival_s=0.05), dict(C, arrival_s=0.0), dict(D, arrival_s=0.1)]
+    res, st = serve(mix, precision=precision, chunk_steps=50)
+    same(alone[0], res[1])
+    assert st["completed"] == 4 and st["peak_resident"] >= 2
+    for r in res:
+        assert r["status"] == "budget"
+        assert r["finish_s"] >= r["admit_s"] >= r["arrival_s"]
+
+
+def test_converged_result_same_alone_and_in_mix():
+    a = dict(A, nu_tol=0.5, max_steps=5000)
+    alone, _ = serve([a], chunk_steps=50)
+    assert alone[0]["status"] == "converged" and alone[0]["steps"] % 100 == 0
+    res, _ = serve([B, a, C], chunk_steps=50)
+    same(alone[0], res[1])
+
+
+def test_admission_by_hbm_budget():
+    need = bytes_required(make_params(16, 16, 16))
+    reqs = [req(16, 16, 16, 1e5, s, max_steps=200) for s in range(4)]
+    res, st = serve(reqs, chunk_steps=50, hbm_budget=2 * need + need // 2)
+    assert st["peak_resident"] == 2
+    assert st["completed"] == 4
+    admits = sorted(r["admit_s"] for r in res)
+    assert admits[2] >= min(r["finish_s"] for r in res)  # third admitted only after a retirement
+
+
+def test_reject_divergence_and_time_limit():
+    big = req(64, 64, 64, 1e5, 9, max_steps=100)
+    blowup = req(12, 12, 12, 1e5, 10, amp=1e4, max_steps=400)  # absurd perturbation: non-finite quickly
+    ok = req(12, 12, 12, 1e5, 11, max_steps=200)
+    need_small = bytes_required(make_params(12, 12, 12))
+    res, st = serve([big, blowup, ok], chunk_steps=50, hbm_budget=3 * need_small)
+    assert res[0]["status"] == "rejected"
+    assert res[1]["status"] == "diverged"
+    assert res[2]["status"] == "budget"
+    alone, _ = serve([ok], chunk_steps=50)
+    same(alone[0], res[2])
+    # a time limit shorter than the work: admitted ones end unfinished, late ones pending
+    slow = [req(24, 24, 24, 1e5, 20, max_steps=10 ** 9), req(24, 24, 24, 1e5, 21, arrival=30.0)]
+    res, st = serve(slow, chunk_steps=50, time_limit_s=0.5)
+    assert res[0]["status"] == "unfinished" and res[0]["steps"] > 0
+    assert res[1]["status"] == "pending"
+    assert st["unfinished"] == 1 and st["pending"] == 1
