@@ -148,6 +148,124 @@ def run_ensemble_workload(args) -> dict:
     return out
 
 
+def serve_requests(rate: float, arrivals_s: float, draw_seed: int = 5):
+    """Poisson arrivals at `rate` per second during `arrivals_s` seconds; cube
+    side log-uniform in [32, 256], Ra log-uniform in [1e4, 1e8], Pr from
+    {0.71, 1.0, 7.0}; stop when the running Nu moves < 1e-3 over 1000 steps,
+    or at 50k steps."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(draw_seed)
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > arrivals_s:
+            break
+        side = int(round(math.exp(rng.uniform(math.log(32), math.log(256)))))
+        reqs.append(dict(lx=side, ly=side, lz=side, rayleigh=float(10 ** rng.uniform(4.0, 8.0)),
+                         prandtl=float(rng.choice([0.71, 1.0, 7.0])), seed=5000 + len(reqs), amp=1.0e-3,
+                         arrival_s=t, max_steps=50000, nu_tol=1.0e-3, window=1000))
+    return reqs
+
+
+def run_serve_workload(args) -> dict:
+    from lbm_ddf_amd import Cavity, serve
+
+    reqs = serve_requests(args.serve_rate, args.serve_arrivals)
+    cells = [r["lx"] ** 3 for r in reqs]
+    log(f"serve: {len(reqs)} requests over {args.serve_arrivals:.0f} s, mean {sum(cells) / max(1, len(cells)) / 1e6:.2f} M "
+        f"cells, precision {args.serve_precision}, time limit {args.serve_time_limit:.0f} s")
+    res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
+                    hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit)
+    log(f"serve: {st['completed']} completed, {st['unfinished']} unfinished, {st['pending']} pending, "
+        f"{st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
+        f"p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; peak resident {st['peak_resident']}")
+    # sequential single-cavity rate over the same request sizes (bounded sample)
+    tot, upd = 0.0, 0.0
+    for r in reqs[: args.serve_seq_sample]:
+        c = Cavity(r["lx"], r["ly"], r["lz"], rayleigh=r["rayleigh"], prandtl=r["prandtl"], t_hot=0.5, t_cold=-0.5,
+                   precision=args.serve_precision)
+        c.init_conductive(r["seed"], r["amp"])
+        c.step(4)
+        c.synchronize()
+        t1 = time.perf_counter()
+        c.step(200)
+        c.synchronize()
+        tot += time.perf_counter() - t1
+        upd += r["lx"] ** 3 * 200
+        c.close()
+    seq = upd / tot / 1e6 if tot > 0 else None
+    done = [r for r in res if r["status"] in ("converged", "budget")]
+    offered = sum(r["lx"] ** 3 * 50000 for r in reqs) / args.serve_arrivals / 1e6
+    return {"requests": len(reqs), "stats": st, "sequential_mlups": seq,
+            "speedup_vs_sequential": (st["mlups"] / seq) if seq else None,
+            "offered_mlups_at_step_budget": offered,
+            "completed_by_status": {s: sum(1 for r in res if r["status"] == s)
+                                    for s in ("converged", "budget", "diverged", "rejected", "unfinished", "pending")},
+            "completed_steps_mean": (sum(r["steps"] for r in done) / len(done)) if done else None}
+
+
+def batch_means(x, nbatch: int = 10):
+    """Mean and 95% half-width from batch means (serially correlated samples)."""
+    import numpy as np
+
+    x = np.asarray(x, dtype=np.float64)
+    b = np.array_split(x, nbatch)
+    m = np.array([v.mean() for v in b if len(v)])
+    half = 2.262 * m.std(ddof=1) / np.sqrt(len(m)) if len(m) > 1 else float("nan")  # t(0.975, 9)
+    return float(x.mean()), float(half)
+
+
+def run_precision_workload(args) -> dict:
+    """One precision of the reduced-precision study: L^3 cavity at Ra, the
+    diagnostics folded in every `sample` steps, Nu time-averaged over
+    [avg_from, steps]; a strided T snapshot is written for max|dT| across
+    precisions."""
+    import numpy as np
+
+    from lbm_ddf_amd import Cavity
+
+    L, prec = args.size, args.precision
+    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
+    c.init_conductive(3, 1.0e-3)
+    if args.autotune > 0:
+        c.autotune(args.autotune)
+        # the autotuner advanced the state; restart from the same seeded state
+        c.init_conductive(3, 1.0e-3)
+    recs = []
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 2000
+    while done < args.steps:
+        n = min(chunk, args.steps - done)
+        recs += c.run_diag(n, args.sample)
+        done += n
+        el = time.perf_counter() - t0
+        log(f"precision {prec}: step {done}/{args.steps}, {el:.0f} s, Nu_bulk {recs[-1]['nu_bulk']:.4f}, "
+            f"mass {recs[-1]['mass']:.3e}")
+    wall = time.perf_counter() - t0
+    phi = c.get_field("phi")
+    snap = phi[::args.snapshot_stride, ::args.snapshot_stride, ::args.snapshot_stride].astype(np.float64)
+    out_dir = ROOT / "gpurun_out"
+    out_dir.mkdir(exist_ok=True)
+    np.save(out_dir / f"precision_T_{prec}_{L}.npy", snap)
+    c.close()
+    sel = [r for r in recs if r["step"] > args.avg_from]
+    res = {"precision": prec, "size": L, "ra": args.ra, "steps": args.steps, "samples": len(sel),
+           "mlups": L ** 3 * args.steps / wall / 1e6,
+           "mass_start": recs[0]["mass"], "mass_end": recs[-1]["mass"],
+           "mass_drift_per_cell": (recs[-1]["mass"] - recs[0]["mass"]) / L ** 3,
+           "nonfinite": recs[-1]["nonfinite"]}
+    for k in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+        m, h = batch_means([r[k] for r in sel])
+        res[k] = {"mean": m, "ci95": h}
+    res["series"] = [(r["step"], r["nu_bulk"], r["nu_top"], r["nu_bottom"], r["mass"], r["kinetic_energy"])
+                     for r in recs]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,12 +279,22 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "serve", "precision"],
                     help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
     ap.add_argument("--sequential-steps", type=int, default=300)
     ap.add_argument("--tuning-cache", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-arrivals", type=float, default=60.0, help="seconds during which requests arrive")
+    ap.add_argument("--serve-time-limit", type=float, default=120.0)
+    ap.add_argument("--serve-precision", default="fp16", choices=["fp64", "fp16", "bf16"])
+    ap.add_argument("--serve-chunk", type=int, default=100)
+    ap.add_argument("--serve-budget-gb", type=float, default=250.0)
+    ap.add_argument("--serve-seq-sample", type=int, default=16)
+    ap.add_argument("--sample", type=int, default=100, help="precision workload: steps between Nu samples")
+    ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
+    ap.add_argument("--snapshot-stride", type=int, default=4)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
                          "the HBM-counter calibration run")
@@ -184,6 +312,29 @@ def main() -> int:
         print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
               "note (DESIGN.md), not an implemented path. Run with --gpus 1.", file=sys.stderr)
         return 2
+
+    if args.workload == "precision":
+        res = run_precision_workload(args)
+        (ROOT / "gpurun_out").mkdir(exist_ok=True)
+        (ROOT / "gpurun_out" / f"precision_{args.precision}_{args.size}.json").write_text(json.dumps(res))
+        print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
+        return 0
+
+    if args.workload == "serve":
+        res = run_serve_workload(args)
+        st = res["stats"]
+        print(json.dumps({
+            "metric": "aggregate MLUPS of the continuous-batching request loop, with p50/p99 time to result",
+            "value": round(st["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": None, "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64" if args.serve_precision == "fp64" else f"{args.serve_precision} storage, f32 arithmetic",
+            "data": f"synthetic: Poisson arrivals {args.serve_rate:g}/s for {args.serve_arrivals:g} s, seed 5",
+            "config": {"workload": "RB cavities, side log-U[32,256], Ra log-U[1e4,1e8], Pr {0.71,1,7}, "
+                                   "stop |dNu|<1e-3 per 1000 steps or 50k steps, HBM budget "
+                                   f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit},
+            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "serve": res,
+        }), flush=True)
+        return 0
 
     if args.workload == "ensemble":
         res = run_ensemble_workload(args)

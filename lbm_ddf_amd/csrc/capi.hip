@@ -147,7 +147,7 @@ int finalize(lbm_ctx *c, bool reduce, int64_t slot) {
     if (c->phase != PHASE_FUSED) return LBM_OK;
     int rc = run_pass(c, lbm::MODE_FINALIZE, reduce);
     if (rc) return rc;
-    if (reduce) LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev + slot, c->stream));
+    if (reduce) LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev + slot, c->partials2, c->stream));
     c->phase = PHASE_REF;
     return LBM_OK;
 }
@@ -237,7 +237,7 @@ int lbm_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->f[0], c->f[1], c->g[0], c->g[1], c->rho, c->ux, c->uy, c->uz, c->phi, c->fy,
-                    c->lag, c->wall_rho, c->partials, c->diag_dev, c->plane};
+                    c->lag, c->wall_rho, c->partials, c->partials2, c->diag_dev, c->plane};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -320,6 +320,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
         {&c->lag, n * side_size(c->prec)},
         {&c->wall_rho, 2ull * p->lx * p->lz * side_size(c->prec)},
         {reinterpret_cast<void **>(&c->partials), max_partials_bytes(p)},
+        {reinterpret_cast<void **>(&c->partials2), static_cast<size_t>(lbm::kPartialBlocks) * lbm::NQ * 8},
     };
     for (auto &al : allocs) {
         if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
@@ -462,7 +463,7 @@ int lbm_run_diag(lbm_ctx *c, int64_t nsteps, int64_t every, lbm_diag *out, int64
             const bool red = s > start && s % every == 0 && nrec < cap;
             if (int rc = run_pass(c, lbm::MODE_FUSED, red)) return rc;
             if (red) {
-                LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, s), c->diag_dev + nrec, c->stream));
+                LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, s), c->diag_dev + nrec, c->partials2, c->stream));
                 ++nrec;
             }
         }
@@ -489,7 +490,7 @@ int lbm_reduce(lbm_ctx *c, lbm_diag *out) {
     if (!out) return fail(LBM_E_INVALID, "out is NULL");
     if (int rc = ensure_ref(c)) return rc;
     LBM_HIP(lbm::launch_reduce_fields(make_args(c, nullptr, nullptr, nullptr, nullptr), c->bx, c->stream));
-    LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev, c->stream));
+    LBM_HIP(lbm::launch_diag_finish(c->partials, nblocks(c), diag_args(c, c->nstep), c->diag_dev, c->partials2, c->stream));
     LBM_HIP(hipMemcpyAsync(out, c->diag_dev, sizeof(lbm_diag), hipMemcpyDeviceToHost, c->stream));
     LBM_HIP(hipStreamSynchronize(c->stream));
     return LBM_OK;

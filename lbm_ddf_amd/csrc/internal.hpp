@@ -75,8 +75,11 @@ hipError_t launch_macrovar(const StepArgs &a, hipStream_t s);
 // Diagnostics over the reference-state fields; the finish kernel turns the
 // per-block partials into an lbm_diag at dst.
 hipError_t launch_reduce_fields(const StepArgs &a, int bx, hipStream_t s);
+// Large partial arrays are first summed by kPartialBlocks blocks into
+// scratch (kPartialBlocks * NQ doubles); nullptr scratch: one pass.
+constexpr int kPartialBlocks = 1024;
 hipError_t launch_diag_finish(const double *partials, int64_t nblocks, const DiagArgs &d,
-                              lbm_diag *dst, hipStream_t s);
+                              lbm_diag *dst, double *scratch, hipStream_t s);
 hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s);
 
 // Plain device copy of bytes (multiple of 16) for the bandwidth reference.
@@ -121,6 +124,7 @@ struct lbm_ctx {
     void *lag = nullptr;
     void *wall_rho = nullptr;
     double *partials = nullptr;
+    double *partials2 = nullptr;  // first-level sums (kPartialBlocks x NQ)
     lbm_diag *diag_dev = nullptr;
     int64_t diag_cap = 0;
     double *plane = nullptr;  // one population plane in FP64 (codec staging)

@@ -242,6 +242,17 @@ __global__ __launch_bounds__(kFinishThreads) void diag_finish_kernel(const doubl
     diag_finish_block(partials, nblocks, d, dst);
 }
 
+// First level for large grids: block b sums records [b*per, (b+1)*per) into
+// out[b].  The split depends only on nblocks, so the result is deterministic.
+__global__ __launch_bounds__(kFinishThreads) void partial_sum_kernel(const double *partials, int64_t nblocks,
+                                                                    int64_t per, double *out) {
+    __shared__ double sh[kFinishThreads][NQ + 1];
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * per;
+    const int64_t count = first < nblocks ? (nblocks - first < per ? nblocks - first : per) : 0;
+    sum_records_block(partials + first * NQ, count, sh);
+    if (threadIdx.x < NQ) out[static_cast<int64_t>(blockIdx.x) * NQ + threadIdx.x] = sh[0][threadIdx.x];
+}
+
 // output_profile, src/output.cu:336-358: one block per y-plane.
 constexpr int kProfileThreads = 256;
 __global__ __launch_bounds__(kProfileThreads) void profile_kernel(StepArgs a, double *out) {
@@ -390,8 +401,16 @@ hipError_t launch_reduce_fields(const StepArgs &a, int bx, hipStream_t s) {
 }
 
 hipError_t launch_diag_finish(const double *partials, int64_t nblocks, const DiagArgs &d, lbm_diag *dst,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(kFinishThreads), 0, s, partials, nblocks, d, dst);
+                              double *scratch, hipStream_t s) {
+    if (scratch && nblocks > 8 * kPartialBlocks) {
+        const int64_t per = (nblocks + kPartialBlocks - 1) / kPartialBlocks;
+        hipLaunchKernelGGL(partial_sum_kernel, dim3(kPartialBlocks), dim3(kFinishThreads), 0, s, partials, nblocks, per,
+                           scratch);
+        hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(kFinishThreads), 0, s, scratch,
+                           static_cast<int64_t>(kPartialBlocks), d, dst);
+    } else {
+        hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(kFinishThreads), 0, s, partials, nblocks, d, dst);
+    }
     return hipGetLastError();
 }
 

@@ -164,9 +164,11 @@ __device__ __forceinline__ void diag_terms(double (&q)[NQ], bool active, int iy,
 // Sums per-block partials in a fixed order (one block of kFinishThreads) and
 // applies the formulas of src/output.cu:228-256 (Nu) and :31-34 (means).
 constexpr int kFinishThreads = 256;
-__device__ __forceinline__ void diag_finish_block(const double *partials, int64_t nblocks, const DiagArgs &d,
-                                                  lbm_diag *dst) {
-    __shared__ double sh[kFinishThreads][NQ + 1];
+// Fixed-order sum of n partial records by one block of kFinishThreads:
+// thread t takes records t, t+256, ...; then a fixed tree.  Totals end in
+// sh[0][*].
+__device__ __forceinline__ void sum_records_block(const double *partials, int64_t nblocks,
+                                                  double (&sh)[kFinishThreads][NQ + 1]) {
     const int t = threadIdx.x;
     double acc[NQ];
 #pragma unroll
@@ -189,7 +191,13 @@ __device__ __forceinline__ void diag_finish_block(const double *partials, int64_
         }
         __syncthreads();
     }
-    if (t == 0) {
+}
+
+__device__ __forceinline__ void diag_finish_block(const double *partials, int64_t nblocks, const DiagArgs &d,
+                                                  lbm_diag *dst) {
+    __shared__ double sh[kFinishThreads][NQ + 1];
+    sum_records_block(partials, nblocks, sh);
+    if (threadIdx.x == 0) {
         const double ncell = static_cast<double>(d.lx) * d.ly * d.lz;
         const double nplane = static_cast<double>(d.lx) * d.lz;
         const double *S = sh[0];
@@ -370,41 +378,54 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         const bool ylo = iy > 0, yhi = iy < ly - 1;
         const bool zlo = iz > 0, zhi = iz < lz - 1;
         const int sy = lx, sz = lx * ly;
-        T rw = T(0);
-        if (wall_lo || wall_hi) rw = static_cast<T>(wall_rho[wall_idx]);
-        const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
+        const bool wall = wall_lo || wall_hi;
+        L rw_raw = L(0);
+        if (wall) rw_raw = wall_rho[wall_idx];
+        // All 54 loads are issued before any value is used, so their
+        // latencies overlap; a direction whose source lies outside the domain
+        // reads the cell's own opposite population instead (half-way
+        // bounce-back), selected by address, not by branch.
+        S fr[Q], gr[Q];
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
             if constexpr (i == 0) {
-                f[0] = P::dec_f(ld<NTL>(&fa[idx]));
-                g[0] = P::template dec_g<0>(ld<NTL>(&ga[idx]), t0);
+                fr[0] = ld<NTL>(&fa[idx]);
+                gr[0] = ld<NTL>(&ga[idx]);
             } else {
                 const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                 const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
                 const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
-                if (inx && iny && inz) {
-                    const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
-                    f[i] = P::dec_f(ld<NTL && cx == 0>(&fa[i * n + src]));
-                    g[i] = P::template dec_g<i>(ld<NTL && cx == 0>(&ga[i * n + src]), t0);
-                } else {
-                    // Half-way bounce-back from the cell's own opposite
-                    // population (f at every wall; g at the x and z walls).
-                    f[i] = P::dec_f(fa[o * n + idx]);
-                    T gv = P::template dec_g<o>(ga[o * n + idx], t0);
-                    if constexpr (cy != 0) {
-                        // Isothermal anti-bounce-back at the y walls; the
-                        // reference applies the z walls after the y walls,
-                        // so an out-of-domain z source keeps bounce-back.
-                        if (!iny && inz) {
-                            constexpr T w2 = static_cast<T>(2.0 * weight(i));
-                            gv = -gv + w2 * tw * (T(1) + rw);
-                        }
-                    }
-                    g[i] = gv;
-                }
+                const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
+                const uint64_t off = (inx && iny && inz) ? i * n + src : o * n + idx;
+                fr[i] = ld<NTL && cx == 0>(&fa[off]);
+                gr[i] = ld<NTL && cx == 0>(&ga[off]);
             }
         });
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            f[i] = P::dec_f(fr[i]);
+            g[i] = P::template dec_g<i>(gr[i], t0);  // w_opp(i) = w_i: same shift either way
+        });
+        if (wall) {
+            // Isothermal anti-bounce-back at the y walls for g.  The reference
+            // applies the z walls after the y walls, so a direction whose
+            // source is also outside in z keeps the adiabatic bounce-back.
+            const T rw = static_cast<T>(rw_raw);
+            const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cy = CY[i], cz = CZ[i];
+                if constexpr (cy != 0) {
+                    const bool iny = cy > 0 ? ylo : yhi;
+                    const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+                    if (!iny && inz) {
+                        constexpr T w2 = static_cast<T>(2.0 * weight(i));
+                        g[i] = -g[i] + w2 * tw * (T(1) + rw);
+                    }
+                }
+            });
+        }
         // ---- moments, summed in direction order (src/macrovar.cu:17-27)
         T jx = T(0), jy = T(0), jz = T(0);
         rho = T(0);
