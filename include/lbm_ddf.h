@@ -175,6 +175,19 @@ int lbm_autotune(lbm_ctx *ctx, int32_t steps, int64_t *steps_taken, double *best
  * reference for the roofline fraction. */
 int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps);
 
+/* ---- Checkpoints in the reference's binary formats ------------------------
+ * lbm_output_flow: output_flow (src/output.cu:67-146): rho', u_x, u_y, u_z,
+ *   phi, each LX*LY*LZ doubles, x fastest, native byte order.
+ * lbm_output_fg:   output_fg (src/output.cu:379-419): f then g, 27*LX*LY*LZ
+ *   doubles each, [ip][iz][iy][ix].
+ * lbm_restart:     the CONTINUE_STEP branch of initialize()
+ *   (src/initial.cu:110-205): reads both files, recomputes F from phi
+ *   (:202) and sets the step count; the run then continues bit for bit.
+ * Files always hold FP64 values (16-bit contexts convert). */
+int lbm_output_flow(lbm_ctx *ctx, const char *path);
+int lbm_output_fg(lbm_ctx *ctx, const char *path);
+int lbm_restart(lbm_ctx *ctx, const char *flow_path, const char *fg_path, int64_t step);
+
 /* ---- Ensembles: many cavities, one launch per step ------------------------
  * No reference counterpart (the reference runs one cavity per process).
  * Cavities of different size, Ra, Pr and walls advance together: one

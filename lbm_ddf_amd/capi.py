@@ -95,6 +95,9 @@ SIGNATURES = {
     "lbm_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "lbm_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _D]),
     "lbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, _D]),
+    "lbm_output_flow": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "lbm_output_fg": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "lbm_restart": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]),
     "lbm_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_P)]),
     "lbm_ensemble_destroy": (ctypes.c_int, [_P]),
     "lbm_ensemble_add": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_double,
@@ -308,6 +311,19 @@ class Cavity:
     def set_pops(self, which: str, values) -> None:
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (27,) + self.shape))
         _check(self.lib.lbm_set_pops(self.h, POPS[which], _dptr(a)))
+
+    # -- checkpoints (reference binary formats) ----------------------------
+    def output_flow(self, path) -> None:
+        """output_flow (src/output.cu:67-146): rho', ux, uy, uz, phi as FP64."""
+        _check(self.lib.lbm_output_flow(self.h, os.fsencode(path)))
+
+    def output_fg(self, path) -> None:
+        """output_fg (src/output.cu:379-419): f then g as FP64."""
+        _check(self.lib.lbm_output_fg(self.h, os.fsencode(path)))
+
+    def restart(self, flow_path, fg_path, step: int) -> None:
+        """initialize() with CONTINUE_STEP (src/initial.cu:110-205)."""
+        _check(self.lib.lbm_restart(self.h, os.fsencode(flow_path), os.fsencode(fg_path), int(step)))
 
     def state(self) -> dict:
         """The reference state: f, g, rho', u, phi."""

@@ -270,6 +270,13 @@ int ctx_prologue(lbm_ctx *c) {
 
 int ctx_ensure_ref(lbm_ctx *c) { return ensure_ref(c); }
 
+int ctx_forcing(lbm_ctx *c) {
+    if (int rc = ensure_ref(c)) return rc;
+    LBM_HIP(lbm::launch_forcing(make_args(c, nullptr, nullptr, nullptr, nullptr), c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
 // Creates a context; with stream != nullptr the context borrows that stream
 // (an ensemble's) instead of creating its own.
 int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {

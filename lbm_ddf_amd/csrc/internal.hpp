@@ -145,5 +145,6 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out);
 StepArgs ctx_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, void *gb);
 int ctx_prologue(lbm_ctx *c);    // REF -> FUSED (collides the next step)
 int ctx_ensure_ref(lbm_ctx *c);  // FUSED -> REF (finalize)
+int ctx_forcing(lbm_ctx *c);     // forcing kernel on the reference state (any precision)
 int ctx_fail(int code, const std::string &msg);
 }  // namespace lbm
