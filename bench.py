@@ -178,7 +178,8 @@ def run_serve_workload(args) -> dict:
     log(f"serve: {len(reqs)} requests over {args.serve_arrivals:.0f} s, mean {sum(cells) / max(1, len(cells)) / 1e6:.2f} M "
         f"cells, precision {args.serve_precision}, time limit {args.serve_time_limit:.0f} s")
     res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
-                    hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit)
+                    hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit,
+                    max_resident_cells=int(args.serve_max_cells))
     log(f"serve: {st['completed']} completed, {st['unfinished']} unfinished, {st['pending']} pending, "
         f"{st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
         f"p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; peak resident {st['peak_resident']}")
@@ -292,6 +293,8 @@ def main() -> int:
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     ap.add_argument("--serve-seq-sample", type=int, default=16)
+    ap.add_argument("--serve-max-cells", type=float, default=64e6,
+                    help="resident-cells cap of the admission (0: HBM budget only)")
     ap.add_argument("--sample", type=int, default=100, help="precision workload: steps between Nu samples")
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)

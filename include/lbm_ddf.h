@@ -272,6 +272,9 @@ typedef struct lbm_serve_options {
     uint64_t hbm_budget;  /* bytes for resident cavities */
     double time_limit_s;
     int32_t block_x, blocks_per_cu, flags; /* ensemble launch shape; block_x 0: default */
+    int32_t reserved;
+    uint64_t max_resident_cells; /* 0: no cap; else admit while resident cells stay below it
+                                    (enough cells to saturate HBM, so admitted requests finish) */
 } lbm_serve_options;
 
 typedef struct lbm_serve_stats {

@@ -83,7 +83,7 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
     size_t next_arrival = 0;
     std::vector<Live> live;
     std::vector<lbm_diag> diag;
-    uint64_t bytes_used = 0, peak_bytes = 0;
+    uint64_t bytes_used = 0, peak_bytes = 0, cells_used = 0;
     int32_t peak_resident = 0;
     double cell_updates = 0.0, busy = 0.0;
     int rc = LBM_OK;
@@ -103,6 +103,7 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
         r.nu_running = l.run_nu;
         (void)lbm_ensemble_remove(ens, l.slot);
         bytes_used -= l.bytes;
+        cells_used -= l.cells;
         l.req = -1;
     };
 
@@ -124,8 +125,11 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
                 waiting.erase(waiting.begin() + w);
                 continue;
             }
+            const uint64_t cells = static_cast<uint64_t>(q.lx) * q.ly * q.lz;
             const bool room = bytes_used + need <= opt->hbm_budget &&
-                              (opt->max_resident <= 0 || static_cast<int32_t>(live.size()) < opt->max_resident);
+                              (opt->max_resident <= 0 || static_cast<int32_t>(live.size()) < opt->max_resident) &&
+                              (opt->max_resident_cells == 0 || live.empty() ||
+                               cells_used + cells <= opt->max_resident_cells);
             if (!room) {
                 ++w;
                 continue;
@@ -133,10 +137,11 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
             Live l;
             l.req = id;
             l.bytes = need;
-            l.cells = static_cast<uint64_t>(q.lx) * q.ly * q.lz;
+            l.cells = cells;
             if ((rc = lbm_ensemble_add(ens, &p, q.seed, q.amp, &l.slot)) != LBM_OK) goto out;
             l.next_check = q.window;
             bytes_used += need;
+            cells_used += cells;
             results[id].admit_s = seconds_since(t0);
             results[id].status = LBM_REQ_UNFINISHED;
             live.push_back(l);

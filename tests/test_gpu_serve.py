@@ -63,6 +63,14 @@ def test_admission_by_hbm_budget():
     assert admits[2] >= min(r["finish_s"] for r in res)  # third admitted only after a retirement
 
 
+def test_resident_cells_cap():
+    reqs = [req(16, 16, 16, 1e5, s, max_steps=200) for s in range(4)]
+    res, st = serve(reqs, chunk_steps=50, max_resident_cells=2 * 16 ** 3)
+    assert st["peak_resident"] == 2 and st["completed"] == 4
+    alone, _ = serve([reqs[3]], chunk_steps=50)
+    same(alone[0], res[3])
+
+
 def test_reject_divergence_and_time_limit():
     big = req(64, 64, 64, 1e5, 9, max_steps=100)
     blowup = req(12, 12, 12, 1e5, 10, amp=1e4, max_steps=400)  # absurd perturbation: non-finite quickly
