@@ -138,10 +138,28 @@ def run_ensemble_workload(args) -> dict:
             cv.close()
         seq = cells * seq_steps / tot / 1e6
         log(f"sequential {prec}: {seq:.0f} MLUPS ({seq_steps} steps per cavity, {tot:.2f} s)")
+        # the reference's execution structure (stage launches + 3 syncs per step), FP64 only
+        ref_seq = None
+        if prec == "fp64":
+            tot_r, upd_r = 0.0, 0.0
+            for c in cavs:
+                cv = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], t_hot=0.5,
+                            t_cold=-0.5)
+                cv.init_conductive(c["seed"], 1.0e-3)
+                cv.step_staged(2)
+                t1 = time.perf_counter()
+                cv.step_staged(args.staged_steps)
+                tot_r += time.perf_counter() - t1
+                upd_r += c["lx"] * c["ly"] * c["lz"] * args.staged_steps
+                cv.close()
+            ref_seq = upd_r / tot_r / 1e6
+            log(f"sequential {prec}, reference structure: {ref_seq:.0f} MLUPS")
         out["precisions"][prec] = {
             "aggregate_mlups": round(agg, 1), "ms_per_step": round(wall / args.ensemble_steps * 1e3, 4),
             "event_ms_per_step": round(ms / args.ensemble_steps, 4), "sequential_mlups": round(seq, 1),
             "speedup_vs_sequential": round(agg / seq, 2),
+            "sequential_reference_structure_mlups": round(ref_seq, 1) if ref_seq else None,
+            "speedup_vs_reference_structure": round(agg / ref_seq, 2) if ref_seq else None,
             "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "graph_steps": 32},
             "nonfinite_cavities": bad,
         }
@@ -267,6 +285,44 @@ def run_precision_workload(args) -> dict:
     return res
 
 
+def precision_report(size: int, avg_from: int) -> dict:
+    """Compares the 16-bit runs of the precision workload with the FP64 run
+    (files written by --workload precision under gpurun_out/)."""
+    import numpy as np
+
+    d = ROOT / "gpurun_out"
+    runs = {}
+    for prec in ("fp64", "fp16", "bf16"):
+        f = d / f"precision_{prec}_{size}.json"
+        if f.exists():
+            runs[prec] = json.loads(f.read_text())
+    if "fp64" not in runs:
+        raise SystemExit("precision-report: the FP64 run is missing")
+    ref = runs["fp64"]
+    a_ref = np.array(ref["series"])
+    t_ref = np.load(d / f"precision_T_fp64_{size}.npy")
+    out = {"size": size, "ra": ref["ra"], "steps": ref["steps"], "avg_from": avg_from, "fp64": {
+        k: ref[k] for k in ("nu_bulk", "nu_top", "nu_bottom", "mass_drift_per_cell", "mlups")}}
+    for prec, r in runs.items():
+        if prec == "fp64":
+            continue
+        a = np.array(r["series"])
+        n = min(len(a), len(a_ref))
+        rec = {k: r[k] for k in ("nu_bulk", "nu_top", "nu_bottom", "mass_drift_per_cell", "mlups")}
+        for j, k in ((1, "nu_bulk"), (2, "nu_top"), (3, "nu_bottom")):
+            m = a_ref[:n, 0] > avg_from
+            rec[f"{k}_time_avg_rel_err"] = float(abs(a[:n][m, j].mean() / a_ref[:n][m, j].mean() - 1.0))
+            rec[f"{k}_max_abs_diff_series"] = float(np.max(np.abs(a[:n, j] - a_ref[:n, j])))
+        rec["speedup_vs_fp64"] = r["mlups"] / ref["mlups"]
+        tf = d / f"precision_T_{prec}_{size}.npy"
+        if tf.exists():
+            t = np.load(tf)
+            rec["max_abs_dT_snapshot"] = float(np.max(np.abs(t - t_ref)))
+            rec["rms_dT_snapshot"] = float(np.sqrt(np.mean((t - t_ref) ** 2)))
+        out[prec] = rec
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,12 +336,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "serve", "precision"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "serve", "precision", "precision-report"],
                     help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
     ap.add_argument("--sequential-steps", type=int, default=300)
     ap.add_argument("--tuning-cache", action="store_true")
+    ap.add_argument("--staged-steps", type=int, default=20)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-arrivals", type=float, default=60.0, help="seconds during which requests arrive")
     ap.add_argument("--serve-time-limit", type=float, default=120.0)
@@ -315,6 +372,10 @@ def main() -> int:
         print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
               "note (DESIGN.md), not an implemented path. Run with --gpus 1.", file=sys.stderr)
         return 2
+
+    if args.workload == "precision-report":
+        print(json.dumps(precision_report(args.size, args.avg_from), indent=1), flush=True)
+        return 0
 
     if args.workload == "precision":
         res = run_precision_workload(args)

@@ -131,6 +131,10 @@ int lbm_collision_BGK_scalar(lbm_ctx *ctx);
 int lbm_streaming(lbm_ctx *ctx);
 int lbm_streaming_scalar(lbm_ctx *ctx);
 int lbm_macrovar(lbm_ctx *ctx);
+/* nsteps iterations of src/main.cu:27-48 exactly as the reference issues
+ * them: the six stage launches with a host synchronization after the
+ * collisions, after the streamings and after macrovar (FP64 only). */
+int lbm_step_staged(lbm_ctx *ctx, int64_t nsteps);
 
 /* nsteps iterations of the loop body of src/main.cu:27-48, each one fused
  * collide-and-stream launch for f and g.  Asynchronous on the context's

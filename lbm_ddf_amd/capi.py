@@ -78,6 +78,7 @@ SIGNATURES = {
     "lbm_streaming": (ctypes.c_int, [_P]),
     "lbm_streaming_scalar": (ctypes.c_int, [_P]),
     "lbm_macrovar": (ctypes.c_int, [_P]),
+    "lbm_step_staged": (ctypes.c_int, [_P, ctypes.c_int64]),
     "lbm_step": (ctypes.c_int, [_P, ctypes.c_int64]),
     "lbm_step_timed": (ctypes.c_int, [_P, ctypes.c_int64, _D]),
     "lbm_run_diag": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Diag),
@@ -230,7 +231,8 @@ class Cavity:
         _check(self.lib.lbm_macrovar(self.h))
 
     def staged_step(self, n: int = 1):
-        """n iterations of src/main.cu:27-48 through the stage entry points."""
+        """n iterations of src/main.cu:27-48 through the stage entry points
+        (from Python, one call per stage)."""
         for _ in range(n):
             self.forcing()
             self.collision_BGK()
@@ -238,6 +240,11 @@ class Cavity:
             self.streaming()
             self.streaming_scalar()
             self.macrovar()
+
+    def step_staged(self, n: int = 1):
+        """lbm_step_staged: the reference's loop (six launches, three host
+        synchronizations per step) driven from C++."""
+        _check(self.lib.lbm_step_staged(self.h, int(n)))
 
     # -- fused path -------------------------------------------------------
     def step(self, n: int = 1):

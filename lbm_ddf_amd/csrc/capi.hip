@@ -85,6 +85,8 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
     a.t_hot = c->p.t_hot;
     a.t_cold = c->p.t_cold;
     a.t0 = c->d.t0;
+    a.dtref = (c->p.t_hot - c->p.t_cold) / c->p.ly;
+    a.tref0 = c->p.t_hot - 0.5 * a.dtref;
     a.lag = c->lag;
     a.wall_rho = c->wall_rho;
     a.rho = c->rho;
@@ -95,6 +97,11 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
     a.fy = c->fy;
     a.partials = c->partials;
     return a;
+}
+
+lbm::CodecArgs codec_args(const lbm_ctx *c) {
+    const lbm::StepArgs a = make_args(c, nullptr, nullptr, nullptr, nullptr);
+    return lbm::CodecArgs{a.lx, a.ly, a.tref0, a.dtref};
 }
 
 hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
@@ -413,6 +420,25 @@ int lbm_macrovar(lbm_ctx *c) {
     return LBM_OK;
 }
 
+// The reference's execution structure, src/main.cu:27-48: forcing, both
+// collisions, sync; both streamings, sync; macrovar, sync -- per step.
+int lbm_step_staged(lbm_ctx *c, int64_t nsteps) {
+    LBM_STAGE_PROLOGUE("lbm_step_staged");
+    if (nsteps < 0) return fail(LBM_E_INVALID, "nsteps < 0");
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if (int rc = lbm_forcing(c)) return rc;
+        if (int rc = lbm_collision_BGK(c)) return rc;
+        if (int rc = lbm_collision_BGK_scalar(c)) return rc;
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        if (int rc = lbm_streaming(c)) return rc;
+        if (int rc = lbm_streaming_scalar(c)) return rc;
+        LBM_HIP(hipStreamSynchronize(c->stream));
+        if (int rc = lbm_macrovar(c)) return rc;
+        LBM_HIP(hipStreamSynchronize(c->stream));
+    }
+    return LBM_OK;
+}
+
 // ---- fused path -----------------------------------------------------------
 
 int lbm_step(lbm_ctx *c, int64_t nsteps) {
@@ -677,7 +703,7 @@ int lbm_get_pops(lbm_ctx *c, int32_t which, double *host) {
     const size_t es = elem_size(c->prec);
     for (int ip = 0; ip < 27; ++ip) {
         const void *pl = static_cast<const char *>(src) + static_cast<size_t>(ip) * c->ps * es;
-        LBM_HIP(lbm::launch_decode_plane(c->prec, which, ip, pl, c->plane, c->n, c->d.t0, c->stream));
+        LBM_HIP(lbm::launch_decode_plane(c->prec, which, ip, pl, c->plane, c->n, codec_args(c), c->stream));
         LBM_HIP(hipMemcpyAsync(host + static_cast<size_t>(ip) * c->n, c->plane, sizeof(double) * c->n,
                                hipMemcpyDeviceToHost, c->stream));
     }
@@ -703,7 +729,7 @@ int lbm_set_pops(lbm_ctx *c, int32_t which, const double *host) {
         void *pd = static_cast<char *>(dst) + static_cast<size_t>(ip) * c->ps * es;
         LBM_HIP(hipMemcpyAsync(c->plane, host + static_cast<size_t>(ip) * c->n, sizeof(double) * c->n,
                                hipMemcpyHostToDevice, c->stream));
-        LBM_HIP(lbm::launch_encode_plane(c->prec, which, ip, c->plane, pd, c->n, c->d.t0, c->stream));
+        LBM_HIP(lbm::launch_encode_plane(c->prec, which, ip, c->plane, pd, c->n, codec_args(c), c->stream));
     }
     LBM_HIP(hipStreamSynchronize(c->stream));
     return LBM_OK;

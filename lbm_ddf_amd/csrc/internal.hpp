@@ -43,6 +43,9 @@ struct StepArgs {
     double c1;            // 1 - 0.5/tau (src/collision.cu:31)
     double grav_beta;     // grav0*beta (src/macrovar.cu:45)
     double t_hot, t_cold, t0;
+    // 16-bit storage keeps g_i - w_i*T_ref(iy), with T_ref the conductive
+    // profile between the walls: T_ref(iy) = tref0 - iy*dtref.
+    double tref0, dtref;
     void *lag;            // phi of the previous completed step [n]
     void *wall_rho;       // rho' of the previous completed step at iy=0 and iy=LY-1 [2][lz][lx]
     // reference-state fields (prologue reads, finalize writes)
@@ -86,10 +89,14 @@ hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 
 // Population plane conversion between the storage format and FP64.
+struct CodecArgs {
+    int32_t lx, ly;
+    double tref0, dtref;  // row reference temperature of the g storage
+};
 hipError_t launch_decode_plane(int precision, int which, int ip, const void *src, double *dst,
-                               uint64_t n, double t0, hipStream_t s);
+                               uint64_t n, const CodecArgs &c, hipStream_t s);
 hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst,
-                               uint64_t n, double t0, hipStream_t s);
+                               uint64_t n, const CodecArgs &c, hipStream_t s);
 
 // Population planes are not packed back to back: each plane starts 64
 // elements (512 B in FP64, 128 B in 16-bit) later in its HBM interleave than

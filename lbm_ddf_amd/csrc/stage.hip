@@ -59,14 +59,14 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
     const double fy = a.fy[idx];
     auto *f = static_cast<typename P::S *>(a.fb);
     auto *g = static_cast<typename P::S *>(a.gb);
-    const typename P::T t0 = static_cast<typename P::T>(a.t0);
+    const typename P::T tref = static_cast<typename P::T>(a.tref0 - iy * a.dtref);
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         const double feq = weight(i) * rho * (1.0 - 1.5 * usq);
         const double fv = feq - 0.5 * fy * CY[i] / (1.0 / 3.0) * feq;
         const double geq = weight(i) * phi * (1.0 - 1.5 * usq);
         f[i * n + idx] = P::enc_f(static_cast<typename P::T>(fv));
-        g[i * n + idx] = P::template enc_g<i>(static_cast<typename P::T>(geq), t0);
+        g[i * n + idx] = P::template enc_g<i>(static_cast<typename P::T>(geq), tref);
     });
 }
 
@@ -290,27 +290,33 @@ __global__ __launch_bounds__(kProfileThreads) void profile_kernel(StepArgs a, do
     }
 }
 
+// Row reference temperature of the 16-bit g storage at flat cell index i.
+__device__ __forceinline__ double codec_tref(uint64_t i, const CodecArgs &c) {
+    const uint64_t iy = (i / static_cast<uint64_t>(c.lx)) % static_cast<uint64_t>(c.ly);
+    return c.tref0 - static_cast<double>(iy) * c.dtref;
+}
+
 template <class P, int WHICH, int IP>
-__global__ void decode_kernel(const void *src, double *dst, uint64_t n, double t0) {
+__global__ void decode_kernel(const void *src, double *dst, uint64_t n, CodecArgs c) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const auto s = static_cast<const typename P::S *>(src)[i];
     if constexpr (WHICH == 0) dst[i] = static_cast<double>(P::dec_f(s));
-    else dst[i] = static_cast<double>(P::template dec_g<IP>(s, static_cast<typename P::T>(t0)));
+    else dst[i] = static_cast<double>(P::template dec_g<IP>(s, static_cast<typename P::T>(codec_tref(i, c))));
 }
 
 template <class P, int WHICH, int IP>
-__global__ void encode_kernel(const double *src, void *dst, uint64_t n, double t0) {
+__global__ void encode_kernel(const double *src, void *dst, uint64_t n, CodecArgs c) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const auto v = static_cast<typename P::T>(src[i]);
     auto *d = static_cast<typename P::S *>(dst);
     if constexpr (WHICH == 0) d[i] = P::enc_f(v);
-    else d[i] = P::template enc_g<IP>(v, static_cast<typename P::T>(t0));
+    else d[i] = P::template enc_g<IP>(v, static_cast<typename P::T>(codec_tref(i, c)));
 }
 
 template <class P>
-hipError_t codec_plane(bool encode, int which, int ip, const void *src, void *dst, uint64_t n, double t0,
+hipError_t codec_plane(bool encode, int which, int ip, const void *src, void *dst, uint64_t n, const CodecArgs &t0,
                        hipStream_t s) {
     hipError_t err = hipErrorInvalidValue;
     static_for<Q>([&](auto I) {
@@ -427,7 +433,7 @@ hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s
 }
 
 hipError_t launch_decode_plane(int precision, int which, int ip, const void *src, double *dst, uint64_t n,
-                               double t0, hipStream_t s) {
+                               const CodecArgs &t0, hipStream_t s) {
     switch (precision) {
         case LBM_PREC_FP64: return codec_plane<PolicyF64>(false, which, ip, src, dst, n, t0, s);
         case LBM_PREC_FP16: return codec_plane<PolicyF16>(false, which, ip, src, dst, n, t0, s);
@@ -437,7 +443,7 @@ hipError_t launch_decode_plane(int precision, int which, int ip, const void *src
 }
 
 hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst, uint64_t n,
-                               double t0, hipStream_t s) {
+                               const CodecArgs &t0, hipStream_t s) {
     switch (precision) {
         case LBM_PREC_FP64: return codec_plane<PolicyF64>(true, which, ip, src, dst, n, t0, s);
         case LBM_PREC_FP16: return codec_plane<PolicyF16>(true, which, ip, src, dst, n, t0, s);

@@ -26,7 +26,8 @@
 // FP64 storage keeps the reference's operation order (divisions by RT, tau
 // and tauc included) and is compiled with -ffp-contract=off; the result is
 // then the same as the FP64 oracle bit for bit.  16-bit storage keeps
-// deviations (f as stored by the reference, g - w_i*t0) and computes in FP32.
+// deviations (f as stored by the reference; g - w_i*T_ref(iy) with T_ref the
+// conductive wall-to-wall profile of the cell's row) and computes in FP32.
 #pragma once
 
 #include "internal.hpp"
@@ -53,6 +54,9 @@ struct PolicyF64 {
 
 // FP16: value * 2^15 stored as IEEE binary16 (round to nearest even).  The
 // power-of-two scale keeps the small deviations out of the subnormal range.
+// dec_g/enc_g take the row's reference temperature: g is stored as
+// g_i - w_i*T_ref so that near the isothermal walls, where the Nusselt
+// gradients live, the stored values are small deviations, not |T| ~ 0.5.
 struct PolicyF16 {
     using S = _Float16;
     using T = float;
@@ -63,12 +67,12 @@ struct PolicyF16 {
     __device__ __forceinline__ static T dec_f(S s) { return static_cast<float>(s) * kInv; }
     __device__ __forceinline__ static S enc_f(T t) { return static_cast<_Float16>(t * kScale); }
     template <int I>
-    __device__ __forceinline__ static T dec_g(S s, T t0) {
-        return static_cast<float>(s) * kInv + static_cast<float>(weight(I)) * t0;
+    __device__ __forceinline__ static T dec_g(S s, T tref) {
+        return static_cast<float>(s) * kInv + static_cast<float>(weight(I)) * tref;
     }
     template <int I>
-    __device__ __forceinline__ static S enc_g(T t, T t0) {
-        return static_cast<_Float16>((t - static_cast<float>(weight(I)) * t0) * kScale);
+    __device__ __forceinline__ static S enc_g(T t, T tref) {
+        return static_cast<_Float16>((t - static_cast<float>(weight(I)) * tref) * kScale);
     }
 };
 
@@ -91,12 +95,12 @@ struct PolicyBF16 {
     __device__ __forceinline__ static T dec_f(S s) { return bf16_bits_to_float(s); }
     __device__ __forceinline__ static S enc_f(T t) { return float_to_bf16_bits(t); }
     template <int I>
-    __device__ __forceinline__ static T dec_g(S s, T t0) {
-        return bf16_bits_to_float(s) + static_cast<float>(weight(I)) * t0;
+    __device__ __forceinline__ static T dec_g(S s, T tref) {
+        return bf16_bits_to_float(s) + static_cast<float>(weight(I)) * tref;
     }
     template <int I>
-    __device__ __forceinline__ static S enc_g(T t, T t0) {
-        return float_to_bf16_bits(t - static_cast<float>(weight(I)) * t0);
+    __device__ __forceinline__ static S enc_g(T t, T tref) {
+        return float_to_bf16_bits(t - static_cast<float>(weight(I)) * tref);
     }
 };
 
@@ -353,6 +357,9 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     S *__restrict__ fb = static_cast<S *>(a.fb);
     S *__restrict__ gb = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
+    // row reference temperature of the 16-bit g storage (ignored in FP64)
+    const T dtref = static_cast<T>(a.dtref);
+    const T tref_y = static_cast<T>(a.tref0 - iy * a.dtref);
 
     T f[Q], g[Q];
     T rho, ux, uy, uz, phi;
@@ -365,7 +372,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             f[i] = P::dec_f(ld<NTL>(&fa[i * n + idx]));
-            g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), t0);
+            g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), tref_y);
         });
         rho = static_cast<T>(a.rho[idx]);
         ux = static_cast<T>(a.ux[idx]);
@@ -386,18 +393,21 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         // reads the cell's own opposite population instead (half-way
         // bounce-back), selected by address, not by branch.
         S fr[Q], gr[Q];
+        bool pulled[Q];
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
             if constexpr (i == 0) {
                 fr[0] = ld<NTL>(&fa[idx]);
                 gr[0] = ld<NTL>(&ga[idx]);
+                pulled[0] = true;
             } else {
                 const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                 const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
                 const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
                 const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
-                const uint64_t off = (inx && iny && inz) ? i * n + src : o * n + idx;
+                pulled[i] = inx && iny && inz;
+                const uint64_t off = pulled[i] ? i * n + src : o * n + idx;
                 fr[i] = ld<NTL && cx == 0>(&fa[off]);
                 gr[i] = ld<NTL && cx == 0>(&ga[off]);
             }
@@ -405,7 +415,10 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             f[i] = P::dec_f(fr[i]);
-            g[i] = P::template dec_g<i>(gr[i], t0);  // w_opp(i) = w_i: same shift either way
+            // the value was stored by its source row: iy - c_y when pulled,
+            // the cell's own row when bounced back (w_opp(i) = w_i)
+            const T tref = pulled[i] ? tref_y + static_cast<T>(CY[i]) * dtref : tref_y;
+            g[i] = P::template dec_g<i>(gr[i], tref);
         });
         if (wall) {
             // Isothermal anti-bounce-back at the y walls for g.  The reference
@@ -450,7 +463,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
+                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], tref_y));
             });
             a.rho[idx] = static_cast<double>(rho);
             a.ux[idx] = static_cast<double>(ux);
@@ -468,7 +481,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-            st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], t0));
+            st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], tref_y));
         });
         lag[idx] = static_cast<L>(phi);
         if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);

@@ -196,3 +196,12 @@ def test_reduced_precision_tracks_fp64(prec, tol):
     assert dT < tol, f"max |dT| = {dT}"
     assert abs(d["mass"]) < 1e-1
     assert abs(d["nu_bulk"] - ref.reduce()["nu_bulk"]) < 0.05
+
+
+def test_step_staged_reference_structure_bitwise():
+    """lbm_step_staged (six launches + three syncs per step, src/main.cu:27-48)
+    equals the oracle and the fused path."""
+    c, o = make_pair(11, 7, 6, seed=21, amp=1.0e-2)
+    c.step_staged(9)
+    o.run(9)
+    assert_state_equal(c, o)
