@@ -456,7 +456,10 @@ def main() -> int:
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL[prec] * n / avg_launch_s / 1e9
     key = f"fused_{prec}_{L}"
-    traffic = traffic_from_profiles(key)
+    # traffic was measured per launch variant; fall back to the workload's figure
+    traffic = traffic_from_profiles(f"{key}_bx{c_tuning[0]}_f{c_tuning[1]}")
+    if traffic is None:
+        traffic = traffic_from_profiles(key)
 
     out = {
         "metric": "MLUPS (fused D3Q27 f+g step, one cavity)",
