@@ -331,6 +331,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--ra", type=float, default=1.0e7)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp16", "bf16"])
+    ap.add_argument("--pattern", default="ab", choices=["ab", "aa"],
+                    help="population storage: two buffers (ab) or one, updated in place (aa)")
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -421,9 +423,10 @@ def main() -> int:
     L = args.size
     n = L ** 3
     prec = args.precision
-    need = bytes_required(make_params(L, L, L, precision=prec))
-    log(f"cavity {L}^3 {prec}: {need / 1e9:.1f} GB device memory")
-    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
+    need = bytes_required(make_params(L, L, L, precision=prec, pattern=args.pattern))
+    log(f"cavity {L}^3 {prec} {args.pattern}: {need / 1e9:.1f} GB device memory")
+    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec,
+               pattern=args.pattern)
     c.init_conductive(2, 1.0e-3)
     c.synchronize()
     log(f"warm-up {args.warmup} steps")
@@ -455,7 +458,7 @@ def main() -> int:
     mlups = n * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL[prec] * n / avg_launch_s / 1e9
-    key = f"fused_{prec}_{L}"
+    key = f"fused_{prec}_{L}" + ("_aa" if args.pattern == "aa" else "")
     # traffic was measured per launch variant; fall back to the workload's figure
     traffic = traffic_from_profiles(f"{key}_bx{c_tuning[0]}_f{c_tuning[1]}")
     if traffic is None:
@@ -476,10 +479,13 @@ def main() -> int:
         "data": "synthetic: conductive start with seeded +-1e-3 perturbation",
         "config": {
             "workload": f"Rayleigh-Benard cavity {L}x{L}x{L}, Ra={args.ra:g}, Pr=0.71, walls +-0.5, "
-                        f"{prec} populations, fused collide+stream of f and g",
+                        f"{prec} populations, fused collide+stream of f and g"
+                        + (", one buffer updated in place (AA pattern)" if args.pattern == "aa" else ""),
             "cells": n,
             "parallelism": "single GPU",
             "launch": {"block_x": c_tuning[0], "flags": c_tuning[1], "autotuned": tuned is not None},
+            "pattern": args.pattern,
+            "device_bytes": need,
         },
         "roofline": {
             "bound": "hbm",

@@ -60,8 +60,17 @@ extern "C" {
 #define LBM_POP_F 0
 #define LBM_POP_G 1
 
+/* Population storage pattern (no reference counterpart; the reference
+ * keeps f, g and the streaming targets f_temp, g_temp, src/globals.cu:28-30). */
+#define LBM_PATTERN_AB 0 /* two buffers per distribution, pull streaming */
+#define LBM_PATTERN_AA 1 /* one buffer per distribution, updated in place by
+                            alternating even/odd passes: half the population
+                            memory, same results bit for bit */
+
 /* Physical and lattice parameters: the macros of src/parameters.h:8-26
- * made runtime.  beta <= 0 selects the reference default 0.1/(5*LY). */
+ * made runtime.  beta <= 0 selects the reference default 0.1/(5*LY).
+ * pattern: LBM_PATTERN_* (0, the default of a zero-initialised tail, is the
+ * A/B pattern). */
 typedef struct lbm_params {
     int32_t lx, ly, lz;
     int32_t precision; /* LBM_PREC_* */
@@ -71,6 +80,8 @@ typedef struct lbm_params {
     double t_cold;     /* T_COLD: wall at iy = LY-1 */
     double gravity;    /* GRAVITY */
     double beta;       /* BETA */
+    int32_t pattern;   /* LBM_PATTERN_* */
+    int32_t reserved;  /* must be 0 */
 } lbm_params;
 
 /* Derived parameters, src/globals.cu:11-23. */

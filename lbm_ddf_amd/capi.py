@@ -40,7 +40,11 @@ class Params(ctypes.Structure):
         ("rayleigh", ctypes.c_double), ("prandtl", ctypes.c_double),
         ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double),
         ("gravity", ctypes.c_double), ("beta", ctypes.c_double),
+        ("pattern", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
+
+
+PATTERN = {"ab": 0, "aa": 1}
 
 
 class Derived(ctypes.Structure):
@@ -147,11 +151,12 @@ def _dptr(a: np.ndarray):
 
 
 def make_params(lx, ly, lz, rayleigh=1.0e7, prandtl=0.71, t_hot=1.0, t_cold=0.0, gravity=1.0,
-                beta=0.0, precision="fp64") -> Params:
+                beta=0.0, precision="fp64", pattern="ab") -> Params:
     """Runtime form of the macros of src/parameters.h:8-26 (defaults are the
-    reference's; beta <= 0 selects 0.1/(5*LY))."""
+    reference's; beta <= 0 selects 0.1/(5*LY)).  pattern "ab" keeps two
+    population buffers per distribution, "aa" one (updated in place)."""
     return Params(int(lx), int(ly), int(lz), PREC[precision], float(rayleigh), float(prandtl),
-                  float(t_hot), float(t_cold), float(gravity), float(beta))
+                  float(t_hot), float(t_cold), float(gravity), float(beta), PATTERN[pattern], 0)
 
 
 def derive(params: Params) -> dict:

@@ -1,0 +1,251 @@
+// aa_kernel.hpp -- the step kernel for one population buffer per
+// distribution, updated in place (AA pattern).
+//
+// The pull A/B scheme keeps two copies of f and g.  Here one copy holds them,
+// and two layouts of it alternate.  Slot (i, x) is population plane i at
+// cell x.
+//
+//   N ("natural"):   slot (i, x) holds f_i(x) after streaming -- what the
+//                    reference stores after src/main.cu:41.
+//   E ("exchanged"): slot (opp(j), x) holds the post-collision f*_j(x) when
+//                    x + c_j lies inside the domain; when it does not, the
+//                    slot already holds the wall value f_opp(j)(x) that the
+//                    next streaming would produce there.
+//
+// An even pass reads N at the cell's own slots, collides, and writes E back
+// to the same slots.  An odd pass gathers f_i(x) from slot (opp(i), x - c_i),
+// or from (i, x) where x - c_i lies outside, collides, and writes f*_j(x) to
+// slot (j, x + c_j), or the wall value to (opp(j), x) where x + c_j lies
+// outside -- which is layout N.  In both passes a cell reads and writes the
+// same set of slots and the sets of different cells are disjoint, so the
+// update is race-free in place.
+//
+// The wall rules (src/streaming.cu:3-251) are applied by the writer: the
+// value streamed into (opp(j), x) from outside is a function of the cell's
+// own f*_j(x) and, for the isothermal walls, of rho' of the step just
+// completed -- both known when the cell writes.  The arithmetic is the one of
+// the A/B kernel, so FP64 results are the same bit for bit; no wall-rho side
+// array is needed.
+//
+// 16-bit storage encodes g with the reference temperature of the row of the
+// cell that will read the slot as its own (the destination cell), so every
+// read decodes with the reader's own row, and layout N has the same encoding
+// as the A/B scheme's reference state.
+#pragma once
+
+#include "step_kernel.hpp"
+
+namespace lbm {
+
+template <class P, int MODE, bool REDUCE, int FLAGS>
+__device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
+    constexpr bool NTS = (FLAGS & kNtStore) != 0;
+    constexpr bool NTL = (FLAGS & kNtLoad) != 0;
+    using S = typename P::S;
+    using T = typename P::T;
+    using L = typename P::L;
+    const int lx = a.lx, ly = a.ly, lz = a.lz;
+    const uint64_t n = a.ps;  // population plane stride
+    const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
+    // one buffer per distribution, read and written in place (no __restrict__)
+    S *fs = static_cast<S *>(a.fb);
+    S *gs = static_cast<S *>(a.gb);
+    const T t0 = static_cast<T>(a.t0);
+    const T dtref = static_cast<T>(a.dtref);
+    const T tref_y = static_cast<T>(a.tref0 - iy * a.dtref);
+    const int sy = lx, sz = lx * ly;
+    // neighbour x - c lies inside for c = +1 (lo) / c = -1 (hi); x + c inside
+    // for c = +1 is the hi test and for c = -1 the lo test
+    const bool xlo = ix > 0, xhi = ix < lx - 1;
+    const bool ylo = iy > 0, yhi = iy < ly - 1;
+    const bool zlo = iz > 0, zhi = iz < lz - 1;
+    L *lag = static_cast<L *>(a.lag);
+
+    T f[Q], g[Q];
+    T rho, ux, uy, uz, phi;
+    {
+        S fr[Q], gr[Q];
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
+            if constexpr (MODE == MODE_AA_ODD && i != 0) {
+                const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
+                const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
+                const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+                const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
+                const uint64_t off = (inx && iny && inz) ? o * n + src : i * n + idx;
+                fr[i] = ld<NTL && cx == 0>(&fs[off]);
+                gr[i] = ld<NTL && cx == 0>(&gs[off]);
+            } else {
+                fr[i] = ld<NTL>(&fs[i * n + idx]);
+                gr[i] = ld<NTL>(&gs[i * n + idx]);
+            }
+        });
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            f[i] = P::dec_f(fr[i]);
+            g[i] = P::template dec_g<i>(gr[i], tref_y);
+        });
+    }
+
+    if constexpr (MODE == MODE_AA_PROLOGUE) {
+        rho = static_cast<T>(a.rho[idx]);
+        ux = static_cast<T>(a.ux[idx]);
+        uy = static_cast<T>(a.uy[idx]);
+        uz = static_cast<T>(a.uz[idx]);
+        phi = static_cast<T>(a.phi[idx]);
+    } else {
+        // moments, summed in direction order (src/macrovar.cu:17-27)
+        T jx = T(0), jy = T(0), jz = T(0);
+        rho = T(0);
+        phi = T(0);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            rho = rho + f[i];
+            if constexpr (CX[i] > 0) jx = jx + f[i];
+            if constexpr (CX[i] < 0) jx = jx - f[i];
+            if constexpr (CY[i] > 0) jy = jy + f[i];
+            if constexpr (CY[i] < 0) jy = jy - f[i];
+            if constexpr (CZ[i] > 0) jz = jz + f[i];
+            if constexpr (CZ[i] < 0) jz = jz - f[i];
+            phi = phi + g[i];
+        });
+        const T phi_prev = static_cast<T>(lag[idx]);
+        const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
+        ux = jx;
+        uy = jy + T(0.5) * fprev;
+        uz = jz;
+        if constexpr (MODE == MODE_AA_MOMENTS) {
+            a.rho[idx] = static_cast<double>(rho);
+            a.ux[idx] = static_cast<double>(ux);
+            a.uy[idx] = static_cast<double>(uy);
+            a.uz[idx] = static_cast<double>(uz);
+            a.phi[idx] = static_cast<double>(phi);
+            a.fy[idx] = static_cast<double>(fprev);
+        }
+    }
+
+    if constexpr (MODE != MODE_AA_MOMENTS) {
+        // forcing (src/macrovar.cu:45) and collision
+        const T F = static_cast<T>(a.grav_beta) * (phi - t0);
+        collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
+        const T rw = rho;  // rho' of this step: read by the isothermal walls of the next streaming
+        const T tw = static_cast<T>(iy == 0 ? a.t_hot : a.t_cold);
+        static_for<Q>([&](auto I) {
+            constexpr int j = I;
+            constexpr int cx = CX[j], cy = CY[j], cz = CZ[j], o = OPP[j];
+            if constexpr (j == 0) {
+                st<NTS>(&fs[idx], P::enc_f(f[0]));
+                st<NTS>(&gs[idx], P::template enc_g<0>(g[0], tref_y));
+            } else {
+                const bool dx = cx == 0 ? true : (cx > 0 ? xhi : xlo);
+                const bool dy = cy == 0 ? true : (cy > 0 ? yhi : ylo);
+                const bool dz = cz == 0 ? true : (cz > 0 ? zhi : zlo);
+                const bool inside = dx && dy && dz;
+                T gv = g[j];
+                if constexpr (cy != 0) {
+                    // Direction opp(j) streams into this cell from beyond a y
+                    // wall: isothermal anti-bounce-back, unless its source is
+                    // also beyond a z wall (z walls are applied last).
+                    if (!dy && dz) {
+                        constexpr T w2 = static_cast<T>(2.0 * weight(j));
+                        gv = -gv + w2 * tw * (T(1) + rw);
+                    }
+                }
+                // the slot's reader: cell x + c_j, or this cell for a wall value
+                const T tr = inside ? tref_y - static_cast<T>(cy) * dtref : tref_y;
+                uint64_t dst = o * n + idx;
+                if constexpr (MODE == MODE_AA_ODD) {
+                    if (inside) dst = j * n + (idx + static_cast<uint32_t>(cx + cy * sy + cz * sz));
+                }
+                st<NTS>(&fs[dst], P::enc_f(f[j]));
+                st<NTS>(&gs[dst], P::template enc_g<j>(gv, tr));
+            }
+        });
+        lag[idx] = static_cast<L>(phi);
+    }
+
+    if constexpr (REDUCE && MODE != MODE_AA_PROLOGUE) {
+        diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux), static_cast<double>(uy),
+                   static_cast<double>(uz), static_cast<double>(phi));
+    }
+}
+
+template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
+__global__ __launch_bounds__(BX) void aa_kernel(const StepArgs a) {
+    const int ly = a.ly;
+    const uint32_t b = (FLAGS & kXcdSwizzle) ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t row = b / static_cast<uint32_t>(a.ntx);
+    const int bxi = static_cast<int>(b - row * static_cast<uint32_t>(a.ntx));
+    const int iy = static_cast<int>(row % static_cast<uint32_t>(ly));
+    const int iz = static_cast<int>(row / static_cast<uint32_t>(ly));
+    const int ix = bxi * BX + static_cast<int>(threadIdx.x);
+
+    double q[NQ];
+    if constexpr (REDUCE) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    }
+    if (ix < a.lx) aa_cell<P, MODE, REDUCE, FLAGS>(a, ix, iy, iz, q);
+    if constexpr (REDUCE && MODE != MODE_AA_PROLOGUE)
+        block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(blockIdx.x) * NQ);
+}
+
+template <class P, int MODE, int BXV, int FL>
+hipError_t launch_aa_variant(const StepArgs &a, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((aa_kernel<P, MODE, false, BXV, FL>), grid, dim3(BXV), 0, s, a);
+    return hipGetLastError();
+}
+
+// Launch-shape variants of the even and odd passes: flags 0, 3, 4, 7.
+template <class P, int MODE, int BXV>
+hipError_t launch_aa_flags(const StepArgs &a, int flags, dim3 grid, hipStream_t s) {
+    switch (flags) {
+        case 0: return launch_aa_variant<P, MODE, BXV, 0>(a, grid, s);
+        case 3: return launch_aa_variant<P, MODE, BXV, 3>(a, grid, s);
+        case 4: return launch_aa_variant<P, MODE, BXV, 4>(a, grid, s);
+        case 7: return launch_aa_variant<P, MODE, BXV, 7>(a, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class P, int BXV>
+hipError_t launch_aa_bx(const StepArgs &a, int mode, bool reduce, int flags, dim3 grid, hipStream_t s) {
+    const dim3 block(BXV);
+    switch (mode) {
+        case MODE_AA_EVEN:
+            if (!reduce) return launch_aa_flags<P, MODE_AA_EVEN, BXV>(a, flags, grid, s);
+            hipLaunchKernelGGL((aa_kernel<P, MODE_AA_EVEN, true, BXV, 0>), grid, block, 0, s, a);
+            break;
+        case MODE_AA_ODD:
+            if (!reduce) return launch_aa_flags<P, MODE_AA_ODD, BXV>(a, flags, grid, s);
+            hipLaunchKernelGGL((aa_kernel<P, MODE_AA_ODD, true, BXV, 0>), grid, block, 0, s, a);
+            break;
+        case MODE_AA_PROLOGUE:
+            hipLaunchKernelGGL((aa_kernel<P, MODE_AA_PROLOGUE, false, BXV, 0>), grid, block, 0, s, a);
+            break;
+        case MODE_AA_MOMENTS:
+            if (reduce) hipLaunchKernelGGL((aa_kernel<P, MODE_AA_MOMENTS, true, BXV, 0>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((aa_kernel<P, MODE_AA_MOMENTS, false, BXV, 0>), grid, block, 0, s, a);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_aa_impl(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s) {
+    const int64_t nb = step_grid_blocks(a.lx, a.ly, a.lz, bx);
+    if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (a.fb == nullptr || a.gb == nullptr || a.lag == nullptr) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<uint32_t>(nb));
+    switch (bx) {
+        case 64: return launch_aa_bx<P, 64>(a, mode, reduce, flags, grid, s);
+        case 128: return launch_aa_bx<P, 128>(a, mode, reduce, flags, grid, s);
+        case 256: return launch_aa_bx<P, 256>(a, mode, reduce, flags, grid, s);
+        case 512: return launch_aa_bx<P, 512>(a, mode, reduce, flags, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lbm

@@ -10,6 +10,12 @@
 //   FUSED  populations are post-collision (the streaming half of the last
 //          step is pending), with lag phi and wall rho' valid.
 // Calls that need the reference state run the finalize pass first.
+//
+// With the in-place pattern (LBM_PATTERN_AA) there is one buffer per
+// distribution and the passes alternate between its two layouts
+// (aa_kernel.hpp): REF phase always holds layout N; in FUSED phase aa_layout
+// says whether the next pass is odd (layout E) or even (layout N).  Finalize
+// is the moments pass, preceded by the in-place exchange when the layout is E.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -54,6 +60,8 @@ int validate(const lbm_params *p) {
     if (n >= (1ull << 32)) return fail(LBM_E_INVALID, "cell count must be below 2^32");
     if (p->precision < LBM_PREC_FP64 || p->precision > LBM_PREC_BF16)
         return fail(LBM_E_INVALID, "unknown precision");
+    if (p->pattern != LBM_PATTERN_AB && p->pattern != LBM_PATTERN_AA)
+        return fail(LBM_E_INVALID, "unknown pattern");
     if (!(p->rayleigh > 0.0) || !(p->prandtl > 0.0)) return fail(LBM_E_INVALID, "Ra and Pr must be > 0");
     return LBM_OK;
 }
@@ -105,6 +113,14 @@ lbm::CodecArgs codec_args(const lbm_ctx *c) {
 }
 
 hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
+    if (mode >= lbm::MODE_AA_PROLOGUE) {
+        switch (c->prec) {
+            case LBM_PREC_FP64: return lbm::launch_step_aa_f64(a, mode, reduce, c->bx, c->flags, c->stream);
+            case LBM_PREC_FP16: return lbm::launch_step_aa_f16(a, mode, reduce, c->bx, c->flags, c->stream);
+            case LBM_PREC_BF16: return lbm::launch_step_aa_bf16(a, mode, reduce, c->bx, c->flags, c->stream);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (c->prec) {
         case LBM_PREC_FP64: return lbm::launch_step_f64(a, mode, reduce, c->bx, c->flags, c->stream);
         case LBM_PREC_FP16: return lbm::launch_step_f16(a, mode, reduce, c->bx, c->flags, c->stream);
@@ -113,8 +129,28 @@ hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool 
     }
 }
 
+// One in-place pass: the AA counterpart of the given A/B mode.
+int run_pass_aa(lbm_ctx *c, int mode, bool reduce) {
+    lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[0], c->g[0]);
+    int aa_mode = lbm::MODE_AA_MOMENTS;
+    int next = lbm::AA_LAYOUT_N;
+    if (mode == lbm::MODE_PROLOGUE) {
+        aa_mode = lbm::MODE_AA_PROLOGUE;
+        next = lbm::AA_LAYOUT_E;
+    } else if (mode == lbm::MODE_FUSED) {
+        aa_mode = c->aa_layout == lbm::AA_LAYOUT_E ? lbm::MODE_AA_ODD : lbm::MODE_AA_EVEN;
+        next = c->aa_layout == lbm::AA_LAYOUT_E ? lbm::AA_LAYOUT_N : lbm::AA_LAYOUT_E;
+    } else if (c->aa_layout == lbm::AA_LAYOUT_E) {
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], c->g[0], c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+    }
+    LBM_HIP(launch_step(c, a, aa_mode, reduce));
+    c->aa_layout = next;
+    return LBM_OK;
+}
+
 // One pass of the given mode from the current buffers into the others.
 int run_pass(lbm_ctx *c, int mode, bool reduce) {
+    if (c->aa) return run_pass_aa(c, mode, reduce);
     lbm::StepArgs a = make_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
     LBM_HIP(launch_step(c, a, mode, reduce));
     c->cf = 1 - c->cf;
@@ -210,11 +246,18 @@ bool valid_block(int lx, int bx) {
     return bx == 64 || bx / 2 < lx;  // no block more than half idle
 }
 
+// The in-place passes are built for flags 0, 3, 4 and 7.
+bool valid_flags(bool aa, int flags) {
+    if (flags < 0 || flags > 7) return false;
+    return !aa || flags == 0 || flags == 3 || flags == 4 || flags == 7;
+}
+
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
-    const uint64_t pops = 4ull * 27ull * lbm::plane_stride(n) * elem_size(p->precision);
+    const bool aa = p->pattern == LBM_PATTERN_AA;
+    const uint64_t pops = (aa ? 2ull : 4ull) * 27ull * lbm::plane_stride(n) * elem_size(p->precision);
     const uint64_t fields = 6ull * n * 8ull;
-    const uint64_t side = n * side_size(p->precision) + 2ull * p->lx * p->lz * side_size(p->precision);
+    const uint64_t side = n * side_size(p->precision) + (aa ? 0ull : 2ull * p->lx * p->lz * side_size(p->precision));
     (void)bx;
     return pops + fields + side + max_partials_bytes(p);
 }
@@ -294,6 +337,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
     lbm_ctx *c = new lbm_ctx();
     c->p = *p;
     c->prec = p->precision;
+    c->aa = p->pattern == LBM_PATTERN_AA;
     c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     c->ps = lbm::plane_stride(c->n);
     derive(p, &c->d, &c->beta);
@@ -337,6 +381,8 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
         {reinterpret_cast<void **>(&c->partials2), static_cast<size_t>(lbm::kPartialBlocks) * lbm::NQ * 8},
     };
     for (auto &al : allocs) {
+        // one buffer per distribution and no wall-rho array in place
+        if (c->aa && (al.ptr == &c->f[1] || al.ptr == &c->g[1] || al.ptr == &c->wall_rho)) continue;
         if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
             (void)hipGetLastError();
             lbm_destroy(c);
@@ -361,6 +407,7 @@ int lbm_init_arrays(lbm_ctx *c) {
     lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
     LBM_HIP(lbm::launch_init(a, c->prec, 0, 0, 0.0, c->stream));
     c->phase = PHASE_REF;
+    c->aa_layout = lbm::AA_LAYOUT_N;
     c->nstep = 0;
     return LBM_OK;
 }
@@ -370,6 +417,7 @@ int lbm_init_conductive(lbm_ctx *c, uint64_t seed, double amp) {
     lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
     LBM_HIP(lbm::launch_init(a, c->prec, 1, seed, amp, c->stream));
     c->phase = PHASE_REF;
+    c->aa_layout = lbm::AA_LAYOUT_N;
     c->nstep = 0;
     return LBM_OK;
 }
@@ -401,6 +449,13 @@ int lbm_collision_BGK_scalar(lbm_ctx *c) {
 
 int lbm_streaming(lbm_ctx *c) {
     LBM_STAGE_PROLOGUE("lbm_streaming");
+    if (c->aa) {
+        // in place: each cell moves its populations to the exchanged slots
+        // (with the wall rules), then the pairwise exchange completes the pull
+        LBM_HIP(lbm::launch_aa_flip(make_args(c, nullptr, nullptr, c->f[0], nullptr), false, c->stream));
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], nullptr, c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+        return LBM_OK;
+    }
     LBM_HIP(lbm::launch_stream_f(make_args(c, c->f[c->cf], nullptr, c->f[1 - c->cf], nullptr), c->stream));
     c->cf = 1 - c->cf;
     return LBM_OK;
@@ -408,6 +463,11 @@ int lbm_streaming(lbm_ctx *c) {
 
 int lbm_streaming_scalar(lbm_ctx *c) {
     LBM_STAGE_PROLOGUE("lbm_streaming_scalar");
+    if (c->aa) {
+        LBM_HIP(lbm::launch_aa_flip(make_args(c, nullptr, nullptr, nullptr, c->g[0]), true, c->stream));
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, nullptr, c->g[0], c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+        return LBM_OK;
+    }
     LBM_HIP(lbm::launch_stream_g(make_args(c, nullptr, c->g[c->cg], nullptr, c->g[1 - c->cg]), c->stream));
     c->cg = 1 - c->cg;
     return LBM_OK;
@@ -562,7 +622,8 @@ int lbm_synchronize(lbm_ctx *c) {
 int lbm_set_tuning(lbm_ctx *c, int32_t block_x, int32_t flags) {
     if (int rc = check_ctx(c)) return rc;
     if (!valid_block(c->p.lx, block_x)) return fail(LBM_E_INVALID, "block_x must be 64, 128, 256 or 512 and fit LX");
-    if (flags < 0 || flags > 7) return fail(LBM_E_INVALID, "flags must be in 0..7");
+    if (!valid_flags(c->aa, flags))
+        return fail(LBM_E_INVALID, c->aa ? "flags must be 0, 3, 4 or 7 (in-place pattern)" : "flags must be in 0..7");
     if (block_x != c->bx && c->phase == PHASE_FUSED) {
         // partial-sum layouts differ per block width only for reductions,
         // which are finished within one call; switching is safe at any time.
@@ -594,6 +655,7 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int64_t *steps_taken, double *best_m
     for (int bx : blocks) {
         if (!valid_block(c->p.lx, bx)) continue;
         for (int fl = 0; fl < 8; ++fl) {
+            if (!valid_flags(c->aa, fl)) continue;
             c->bx = bx;
             c->flags = fl;
             // one untimed launch per candidate (code object load, caches)

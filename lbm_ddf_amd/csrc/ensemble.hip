@@ -328,6 +328,8 @@ int lbm_ensemble_destroy(lbm_ensemble *e) {
 int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double amp, int32_t *slot) {
     if (!e || !p || !slot) return ctx_fail(LBM_E_INVALID, "NULL argument");
     if (p->precision != e->prec) return ctx_fail(LBM_E_INVALID, "cavity precision differs from the ensemble's");
+    if (p->pattern != LBM_PATTERN_AB)
+        return ctx_fail(LBM_E_UNSUPPORTED, "ensemble members use the A/B pattern");
     lbm_ctx *c = nullptr;
     if (int rc = lbm::ctx_create(p, e->stream, &c)) return rc;
     if (int rc = lbm_init_conductive(c, seed, amp)) {

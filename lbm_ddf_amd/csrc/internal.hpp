@@ -15,6 +15,11 @@ enum StepMode : int {
     MODE_PROLOGUE = 0,  // forcing + collide f, g of the reference state (no streaming)
     MODE_FUSED = 1,     // stream + moments of step k, forcing + collide of step k+1
     MODE_FINALIZE = 2,  // stream + moments of step k; writes the reference state
+    // In-place (AA) passes over one buffer per distribution:
+    MODE_AA_PROLOGUE = 3,  // layout N + reference fields -> collide -> layout E
+    MODE_AA_EVEN = 4,      // layout N: moments, collide -> layout E (own slots)
+    MODE_AA_ODD = 5,       // layout E: gather, moments, collide -> layout N (neighbour slots)
+    MODE_AA_MOMENTS = 6,   // layout N: moments -> reference fields (no population writes)
 };
 
 // Number of per-block partial sums of the folded diagnostics.
@@ -65,6 +70,16 @@ hipError_t launch_step_f64(const StepArgs &a, int mode, bool reduce, int bx, int
 hipError_t launch_step_f16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
 hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
 
+// In-place (AA) pattern: step passes, the E -> N exchange (any precision; f
+// or g may be null) and the flip of one distribution's post-collision
+// reference state to layout E (FP64, the stage interface's streaming).
+hipError_t launch_step_aa_f64(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
+hipError_t launch_step_aa_f16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
+hipError_t launch_step_aa_bf16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
+hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint64_t ncell, int lx, int ly, int lz,
+                              hipStream_t s);
+hipError_t launch_aa_flip(const StepArgs &a, bool scalar, hipStream_t s);
+
 // Reference-state stage kernels (FP64).
 hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
                        hipStream_t s);
@@ -111,12 +126,15 @@ inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
 }
 
 enum Phase { PHASE_EMPTY = 0, PHASE_REF = 1, PHASE_FUSED = 2 };
+enum AaLayout { AA_LAYOUT_N = 0, AA_LAYOUT_E = 1 };
 
 }  // namespace lbm
 
 // One cavity's device state (opaque to C callers).  Buffers f[2], g[2] hold
 // 27 planes each (stride ps); in REF phase f[cf], g[cg] are the reference's
-// post-streaming populations, in FUSED phase its post-collision ones.
+// post-streaming populations, in FUSED phase its post-collision ones.  With
+// the in-place pattern only f[0], g[0] exist (cf = cg = 0) and aa_layout
+// tells which of the two layouts they hold (always N in REF phase).
 struct lbm_ctx {
     lbm_params p{};
     lbm_derived d{};
@@ -138,6 +156,8 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false when an ensemble lends its stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool aa = false;    // in-place pattern (LBM_PATTERN_AA)
+    int aa_layout = 0;  // AA_LAYOUT_N or AA_LAYOUT_E
     int phase = lbm::PHASE_EMPTY;
     int64_t nstep = 0;
     int bx = 256;

@@ -14,7 +14,8 @@
 //
 // Differences: the time loop calls the fused lbm_step between output events
 // instead of five kernels per step; errors print and return a non-zero exit
-// status; --init conductive selects the seeded conductive start.
+// status; --init conductive selects the seeded conductive start; --pattern aa
+// keeps one population buffer per distribution (in place).
 #include <sys/stat.h>
 
 #include <chrono>
@@ -33,7 +34,7 @@ struct Options {
     int lx = 400, ly = 400, lz = 400;
     long nend = 1000, ndiag = 10000, nflowout = 10000, nnuout = 10000, cont = 0;
     double ra = 1.0e7, pr = 0.71, t_hot = 1.0, t_cold = 0.0, gravity = 1.0, beta = 0.0;
-    std::string precision = "fp64", init = "reference", outdir = ".";
+    std::string precision = "fp64", init = "reference", outdir = ".", pattern = "ab";
     unsigned long long seed = 1;
     double amp = 1.0e-3;
 };
@@ -51,7 +52,8 @@ void usage() {
     std::fprintf(stderr,
                  "rb3d [--lx N --ly N --lz N] [--nend N] [--ndiag N] [--nflowout N] [--nnuout N]\n"
                  "     [--ra X] [--pr X] [--thot X] [--tcold X] [--gravity X] [--beta X]\n"
-                 "     [--precision fp64|fp16|bf16] [--init reference|conductive] [--seed N] [--amp X]\n"
+                 "     [--precision fp64|fp16|bf16] [--pattern ab|aa] [--init reference|conductive]\n"
+                 "     [--seed N] [--amp X]\n"
                  "     [--continue STEP] [--outdir DIR]\n");
 }
 
@@ -77,12 +79,14 @@ bool parse(int argc, char **argv, Options &o) {
         else if (k == "--beta") o.beta = std::atof(v);
         else if (k == "--precision") o.precision = v;
         else if (k == "--init") o.init = v;
+        else if (k == "--pattern") o.pattern = v;
         else if (k == "--seed") o.seed = std::strtoull(v, nullptr, 10);
         else if (k == "--amp") o.amp = std::atof(v);
         else if (k == "--outdir") o.outdir = v;
         else return false;
     }
-    return o.ndiag > 0 && o.nflowout > 0 && o.nnuout > 0 && o.nend >= o.cont;
+    return o.ndiag > 0 && o.nflowout > 0 && o.nnuout > 0 && o.nend >= o.cont &&
+           (o.pattern == "ab" || o.pattern == "aa");
 }
 
 // File suffix of output_nu / output_profile (src/output.cu:164-178).
@@ -105,7 +109,8 @@ int main(int argc, char **argv) {
         return 2;
     }
     int prec = o.precision == "fp16" ? LBM_PREC_FP16 : (o.precision == "bf16" ? LBM_PREC_BF16 : LBM_PREC_FP64);
-    lbm_params p{o.lx, o.ly, o.lz, prec, o.ra, o.pr, o.t_hot, o.t_cold, o.gravity, o.beta};
+    lbm_params p{o.lx, o.ly, o.lz, prec, o.ra, o.pr, o.t_hot, o.t_cold, o.gravity, o.beta,
+                 o.pattern == "aa" ? LBM_PATTERN_AA : LBM_PATTERN_AB, 0};
     char dirname[512];
     std::snprintf(dirname, sizeof dirname, "%s/Ra%.1ePr%.2f", o.outdir.c_str(), o.ra, o.pr);
     mkdir(dirname, 0700);

@@ -56,6 +56,17 @@ def test_bytes_required_fp64_512():
     assert b16 < b / 2.5
 
 
+def test_bytes_required_in_place_pattern():
+    """One population buffer per distribution instead of two: the 512^3 FP64
+    cavity needs about 65 GB instead of 124 GB."""
+    ab = capi.bytes_required(capi.make_params(512, 512, 512))
+    aa = capi.bytes_required(capi.make_params(512, 512, 512, pattern="aa"))
+    n = 512 ** 3
+    ps = ((n + 63) // 64) * 64 + 64
+    assert ab - aa == 2 * 27 * ps * 8 + 2 * 512 * 512 * 8  # second buffers and the wall-rho array
+    assert aa < 66e9
+
+
 @pytest.mark.parametrize("shape", [(4, 2, 4), (0, 8, 8), (8, 8, 0)])
 def test_create_rejects_bad_geometry_without_touching_the_gpu(shape):
     lib = capi.load()
@@ -76,6 +87,10 @@ def test_create_rejects_bad_precision_and_physics():
     p = capi.make_params(8, 8, 8, rayleigh=-1.0)
     assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
     assert b"Ra" in lib.lbm_last_error()
+    p = capi.make_params(8, 8, 8)
+    p.pattern = 2
+    assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"pattern" in lib.lbm_last_error()
 
 
 def test_null_context_is_an_error_not_a_crash():
