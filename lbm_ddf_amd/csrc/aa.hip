@@ -29,20 +29,21 @@ __device__ __forceinline__ void cell_of(uint64_t idx, int lx, int ly, int &ix, i
 }
 
 template <typename E>
-__global__ void aa_exchange_kernel(E *f, E *g, uint64_t ps, uint64_t ncell, int lx, int ly, int lz) {
+__global__ void aa_exchange_kernel(E *f, E *g, uint64_t ps, uint64_t rs, uint64_t ncell, int lx, int ly, int lz) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= ncell) return;
     int ix, iy, iz;
     cell_of(idx, lx, ly, ix, iy, iz);
+    const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * rs + ix;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
         if constexpr (i < o) {
             const int x = ix + cx, y = iy + cy, z = iz + cz;
             if (x >= 0 && x < lx && y >= 0 && y < ly && z >= 0 && z < lz) {
-                const uint64_t a = o * ps + idx;
-                const uint64_t b = i * ps + (idx + static_cast<uint64_t>(static_cast<int64_t>(cx) + static_cast<int64_t>(cy) * lx +
-                                                                        static_cast<int64_t>(cz) * lx * ly));
+                const uint64_t a = at + o * ps;
+                const uint64_t b = at + i * ps + static_cast<uint64_t>(static_cast<int64_t>(cx) +
+                                                                       static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(rs));
                 if (f) {
                     const E t = f[a];
                     f[a] = f[b];
@@ -65,12 +66,13 @@ __global__ void aa_flip_kernel(StepArgs a) {
     int ix, iy, iz;
     cell_of(idx, a.lx, a.ly, ix, iy, iz);
     const int ly = a.ly, lz = a.lz;
-    const uint64_t n = a.ps;
+    const uint64_t ps = a.ps;
+    const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
     double *p = static_cast<double *>(SCALAR ? a.gb : a.fb);
     double v[Q];
     static_for<Q>([&](auto I) {
         constexpr int j = I;
-        v[j] = p[j * n + idx];
+        v[j] = p[at + j * ps];
     });
     const double density = (iy == 0 || iy == ly - 1) ? a.rho[idx] : 0.0;
     const double tw = iy == 0 ? a.t_hot : a.t_cold;
@@ -85,21 +87,21 @@ __global__ void aa_flip_kernel(StepArgs a) {
             // beyond a z wall, which the reference applies last)
             if (!dy && dz) out = -v[j] + 2.0 * weight(j) * tw * (1.0 + density);
         }
-        p[o * n + idx] = out;
+        p[at + o * ps] = out;
     });
 }
 
 }  // namespace
 
-hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint64_t ncell, int lx, int ly, int lz,
-                              hipStream_t s) {
+hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint64_t rs, uint64_t ncell, int lx,
+                              int ly, int lz, hipStream_t s) {
     const dim3 grid(static_cast<uint32_t>((ncell + kFlatBlock - 1) / kFlatBlock));
     if (precision == LBM_PREC_FP64)
         hipLaunchKernelGGL(aa_exchange_kernel<double>, grid, dim3(kFlatBlock), 0, s, static_cast<double *>(f),
-                           static_cast<double *>(g), ps, ncell, lx, ly, lz);
+                           static_cast<double *>(g), ps, rs, ncell, lx, ly, lz);
     else
         hipLaunchKernelGGL(aa_exchange_kernel<uint16_t>, grid, dim3(kFlatBlock), 0, s, static_cast<uint16_t *>(f),
-                           static_cast<uint16_t *>(g), ps, ncell, lx, ly, lz);
+                           static_cast<uint16_t *>(g), ps, rs, ncell, lx, ly, lz);
     return hipGetLastError();
 }
 

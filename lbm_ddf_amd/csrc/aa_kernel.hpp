@@ -2,8 +2,8 @@
 // distribution, updated in place (AA pattern).
 //
 // The pull A/B scheme keeps two copies of f and g.  Here one copy holds them,
-// and two layouts of it alternate.  Slot (i, x) is population plane i at
-// cell x.
+// and two layouts of it alternate.  Slot (i, x) is population i of cell x
+// (row-blocked layout, see pop_stride).
 //
 //   N ("natural"):   slot (i, x) holds f_i(x) after streaming -- what the
 //                    reference stores after src/main.cu:41.
@@ -45,15 +45,16 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     using T = typename P::T;
     using L = typename P::L;
     const int lx = a.lx, ly = a.ly, lz = a.lz;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;  // population stride within a row
     const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
+    // this cell's slot of population 0 (row-blocked layout, see pop_stride)
+    const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
     // one buffer per distribution, read and written in place (no __restrict__)
     S *fs = static_cast<S *>(a.fb);
     S *gs = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
     const T dtref = static_cast<T>(a.dtref);
     const T tref_y = static_cast<T>(a.tref0 - iy * a.dtref);
-    const int sy = lx, sz = lx * ly;
     // neighbour x - c lies inside for c = +1 (lo) / c = -1 (hi); x + c inside
     // for c = +1 is the hi test and for c = -1 the lo test
     const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -72,13 +73,13 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                 const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                 const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
                 const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
-                const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
-                const uint64_t off = (inx && iny && inz) ? o * n + src : i * n + idx;
+                const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
+                const uint64_t off = (inx && iny && inz) ? at + o * ps - static_cast<uint64_t>(d) : at + i * ps;
                 fr[i] = ld<NTL && cx == 0>(&fs[off]);
                 gr[i] = ld<NTL && cx == 0>(&gs[off]);
             } else {
-                fr[i] = ld<NTL>(&fs[i * n + idx]);
-                gr[i] = ld<NTL>(&gs[i * n + idx]);
+                fr[i] = ld<NTL>(&fs[at + i * ps]);
+                gr[i] = ld<NTL>(&gs[at + i * ps]);
             }
         });
         static_for<Q>([&](auto I) {
@@ -135,8 +136,8 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             constexpr int j = I;
             constexpr int cx = CX[j], cy = CY[j], cz = CZ[j], o = OPP[j];
             if constexpr (j == 0) {
-                st<NTS>(&fs[idx], P::enc_f(f[0]));
-                st<NTS>(&gs[idx], P::template enc_g<0>(g[0], tref_y));
+                st<NTS>(&fs[at], P::enc_f(f[0]));
+                st<NTS>(&gs[at], P::template enc_g<0>(g[0], tref_y));
             } else {
                 const bool dx = cx == 0 ? true : (cx > 0 ? xhi : xlo);
                 const bool dy = cy == 0 ? true : (cy > 0 ? yhi : ylo);
@@ -154,9 +155,10 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                 }
                 // the slot's reader: cell x + c_j, or this cell for a wall value
                 const T tr = inside ? tref_y - static_cast<T>(cy) * dtref : tref_y;
-                uint64_t dst = o * n + idx;
+                uint64_t dst = at + o * ps;
                 if constexpr (MODE == MODE_AA_ODD) {
-                    if (inside) dst = j * n + (idx + static_cast<uint32_t>(cx + cy * sy + cz * sz));
+                    const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
+                    if (inside) dst = at + j * ps + static_cast<uint64_t>(d);
                 }
                 st<NTS>(&fs[dst], P::enc_f(f[j]));
                 st<NTS>(&gs[dst], P::template enc_g<j>(gv, tr));

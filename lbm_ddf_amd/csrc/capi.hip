@@ -82,6 +82,7 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
     a.gb = gb;
     a.n = c->n;
     a.ps = c->ps;
+    a.rs = c->rs;
     a.lx = c->p.lx;
     a.ly = c->p.ly;
     a.lz = c->p.lz;
@@ -109,7 +110,7 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
 
 lbm::CodecArgs codec_args(const lbm_ctx *c) {
     const lbm::StepArgs a = make_args(c, nullptr, nullptr, nullptr, nullptr);
-    return lbm::CodecArgs{a.lx, a.ly, a.tref0, a.dtref};
+    return lbm::CodecArgs{a.lx, a.ly, a.rs, a.tref0, a.dtref};
 }
 
 hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
@@ -141,7 +142,8 @@ int run_pass_aa(lbm_ctx *c, int mode, bool reduce) {
         aa_mode = c->aa_layout == lbm::AA_LAYOUT_E ? lbm::MODE_AA_ODD : lbm::MODE_AA_EVEN;
         next = c->aa_layout == lbm::AA_LAYOUT_E ? lbm::AA_LAYOUT_N : lbm::AA_LAYOUT_E;
     } else if (c->aa_layout == lbm::AA_LAYOUT_E) {
-        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], c->g[0], c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], c->g[0], c->ps, c->rs, c->n, c->p.lx, c->p.ly, c->p.lz,
+                                        c->stream));
     }
     LBM_HIP(launch_step(c, a, aa_mode, reduce));
     c->aa_layout = next;
@@ -255,7 +257,7 @@ bool valid_flags(bool aa, int flags) {
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     const bool aa = p->pattern == LBM_PATTERN_AA;
-    const uint64_t pops = (aa ? 2ull : 4ull) * 27ull * lbm::plane_stride(n) * elem_size(p->precision);
+    const uint64_t pops = (aa ? 1ull : 2ull) * lbm::buffer_elems(p->lx, p->ly, p->lz) * elem_size(p->precision);
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + (aa ? 0ull : 2ull * p->lx * p->lz * side_size(p->precision));
     (void)bx;
@@ -286,7 +288,7 @@ int lbm_bytes_required(const lbm_params *p, uint64_t *bytes) {
 int lbm_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->f[0], c->f[1], c->g[0], c->g[1], c->rho, c->ux, c->uy, c->uz, c->phi, c->fy,
+    void *bufs[] = {c->buf[0], c->buf[1], c->rho, c->ux, c->uy, c->uz, c->phi, c->fy,
                     c->lag, c->wall_rho, c->partials, c->partials2, c->diag_dev, c->plane};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -339,7 +341,8 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
     c->prec = p->precision;
     c->aa = p->pattern == LBM_PATTERN_AA;
     c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
-    c->ps = lbm::plane_stride(c->n);
+    c->ps = lbm::pop_stride(p->lx);
+    c->rs = lbm::row_stride(p->lx);
     derive(p, &c->d, &c->beta);
     c->bx = default_block(p->lx);
     hipError_t e = hipGetDevice(&c->device);
@@ -360,15 +363,13 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
         return hip_fail(e, "stream/event creation");
     }
     const uint64_t n = c->n;
-    const size_t pop_bytes = 27ull * c->ps * elem_size(c->prec);
+    const size_t buf_bytes = lbm::buffer_elems(p->lx, p->ly, p->lz) * elem_size(c->prec);
     struct Alloc {
         void **ptr;
         size_t bytes;
     } allocs[] = {
-        {&c->f[0], pop_bytes},
-        {&c->f[1], pop_bytes},
-        {&c->g[0], pop_bytes},
-        {&c->g[1], pop_bytes},
+        {&c->buf[0], buf_bytes},
+        {&c->buf[1], buf_bytes},
         {reinterpret_cast<void **>(&c->rho), n * 8},
         {reinterpret_cast<void **>(&c->ux), n * 8},
         {reinterpret_cast<void **>(&c->uy), n * 8},
@@ -381,13 +382,19 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
         {reinterpret_cast<void **>(&c->partials2), static_cast<size_t>(lbm::kPartialBlocks) * lbm::NQ * 8},
     };
     for (auto &al : allocs) {
-        // one buffer per distribution and no wall-rho array in place
-        if (c->aa && (al.ptr == &c->f[1] || al.ptr == &c->g[1] || al.ptr == &c->wall_rho)) continue;
+        // one buffer and no wall-rho array in place
+        if (c->aa && (al.ptr == &c->buf[1] || al.ptr == &c->wall_rho)) continue;
         if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
             (void)hipGetLastError();
             lbm_destroy(c);
             return fail(LBM_E_NOMEM, "device allocation of " + std::to_string(al.bytes) + " bytes failed");
         }
+    }
+    // f and g of one buffer share its rows: g starts 27 populations later
+    for (int k = 0; k < 2; ++k) {
+        if (!c->buf[k]) continue;
+        c->f[k] = c->buf[k];
+        c->g[k] = static_cast<char *>(c->buf[k]) + 27ull * c->ps * elem_size(c->prec);
     }
     rc = ensure_diag_cap(c, 16);
     if (rc) {
@@ -453,7 +460,8 @@ int lbm_streaming(lbm_ctx *c) {
         // in place: each cell moves its populations to the exchanged slots
         // (with the wall rules), then the pairwise exchange completes the pull
         LBM_HIP(lbm::launch_aa_flip(make_args(c, nullptr, nullptr, c->f[0], nullptr), false, c->stream));
-        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], nullptr, c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, c->f[0], nullptr, c->ps, c->rs, c->n, c->p.lx, c->p.ly, c->p.lz,
+                                        c->stream));
         return LBM_OK;
     }
     LBM_HIP(lbm::launch_stream_f(make_args(c, c->f[c->cf], nullptr, c->f[1 - c->cf], nullptr), c->stream));
@@ -465,7 +473,8 @@ int lbm_streaming_scalar(lbm_ctx *c) {
     LBM_STAGE_PROLOGUE("lbm_streaming_scalar");
     if (c->aa) {
         LBM_HIP(lbm::launch_aa_flip(make_args(c, nullptr, nullptr, nullptr, c->g[0]), true, c->stream));
-        LBM_HIP(lbm::launch_aa_exchange(c->prec, nullptr, c->g[0], c->ps, c->n, c->p.lx, c->p.ly, c->p.lz, c->stream));
+        LBM_HIP(lbm::launch_aa_exchange(c->prec, nullptr, c->g[0], c->ps, c->rs, c->n, c->p.lx, c->p.ly, c->p.lz,
+                                        c->stream));
         return LBM_OK;
     }
     LBM_HIP(lbm::launch_stream_g(make_args(c, nullptr, c->g[c->cg], nullptr, c->g[1 - c->cg]), c->stream));
@@ -753,13 +762,9 @@ int lbm_get_pops(lbm_ctx *c, int32_t which, double *host) {
     if (int rc = check_ctx(c)) return rc;
     if (!host || (which != LBM_POP_F && which != LBM_POP_G)) return fail(LBM_E_INVALID, "bad which/host");
     if (int rc = ensure_ref(c)) return rc;
+    // populations leave in the reference's order [ip][z][y][x], one plane at
+    // a time through a staging buffer (the codec also converts 16-bit storage)
     void *src = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
-    if (c->prec == LBM_PREC_FP64) {
-        LBM_HIP(hipMemcpy2DAsync(host, sizeof(double) * c->n, src, sizeof(double) * c->ps, sizeof(double) * c->n, 27,
-                                 hipMemcpyDeviceToHost, c->stream));
-        LBM_HIP(hipStreamSynchronize(c->stream));
-        return LBM_OK;
-    }
     if (!c->plane && hipMalloc(&c->plane, sizeof(double) * c->n) != hipSuccess)
         return fail(LBM_E_NOMEM, "plane staging buffer");
     const size_t es = elem_size(c->prec);
@@ -778,12 +783,6 @@ int lbm_set_pops(lbm_ctx *c, int32_t which, const double *host) {
     if (!host || (which != LBM_POP_F && which != LBM_POP_G)) return fail(LBM_E_INVALID, "bad which/host");
     if (int rc = ensure_ref(c)) return rc;
     void *dst = which == LBM_POP_F ? c->f[c->cf] : c->g[c->cg];
-    if (c->prec == LBM_PREC_FP64) {
-        LBM_HIP(hipMemcpy2DAsync(dst, sizeof(double) * c->ps, host, sizeof(double) * c->n, sizeof(double) * c->n, 27,
-                                 hipMemcpyHostToDevice, c->stream));
-        LBM_HIP(hipStreamSynchronize(c->stream));
-        return LBM_OK;
-    }
     if (!c->plane && hipMalloc(&c->plane, sizeof(double) * c->n) != hipSuccess)
         return fail(LBM_E_NOMEM, "plane staging buffer");
     const size_t es = elem_size(c->prec);

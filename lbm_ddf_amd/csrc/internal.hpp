@@ -38,10 +38,11 @@ enum DiagQ : int {
 
 // Arguments of the step kernels (one cavity).  Pointers are device pointers.
 struct StepArgs {
-    const void *fa, *ga;  // source populations [27][n]
-    void *fb, *gb;        // destination populations [27][n]
+    const void *fa, *ga;  // source populations (layout: see pop_offset)
+    void *fb, *gb;        // destination populations
     uint64_t n;           // cells
-    uint64_t ps;          // population plane stride in elements (>= n, staggered)
+    uint64_t ps;          // population stride in elements (within a row)
+    uint64_t rs;          // row stride in elements
     int32_t lx, ly, lz;
     int32_t ntx;          // blocks per x-row
     double tau, tauc;     // relaxation times (src/globals.cu:14,17)
@@ -76,8 +77,8 @@ hipError_t launch_step_bf16(const StepArgs &a, int mode, bool reduce, int bx, in
 hipError_t launch_step_aa_f64(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
 hipError_t launch_step_aa_f16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
 hipError_t launch_step_aa_bf16(const StepArgs &a, int mode, bool reduce, int bx, int flags, hipStream_t s);
-hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint64_t ncell, int lx, int ly, int lz,
-                              hipStream_t s);
+hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint64_t rs, uint64_t ncell, int lx,
+                              int ly, int lz, hipStream_t s);
 hipError_t launch_aa_flip(const StepArgs &a, bool scalar, hipStream_t s);
 
 // Reference-state stage kernels (FP64).
@@ -106,6 +107,7 @@ hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s
 // Population plane conversion between the storage format and FP64.
 struct CodecArgs {
     int32_t lx, ly;
+    uint64_t rs;          // row stride of the population layout
     double tref0, dtref;  // row reference temperature of the g storage
 };
 hipError_t launch_decode_plane(int precision, int which, int ip, const void *src, double *dst,
@@ -113,12 +115,27 @@ hipError_t launch_decode_plane(int precision, int which, int ip, const void *src
 hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst,
                                uint64_t n, const CodecArgs &c, hipStream_t s);
 
-// Population planes are not packed back to back: each plane starts 64
-// elements (512 B in FP64, 128 B in 16-bit) later in its HBM interleave than
-// the previous one, so the 54 concurrent streams of a step do not all hit the
-// same channel and bank at the same offset.  Measured on MI355X: +2.5% (FP64)
-// and +12% (16-bit) over tightly packed 2^k-cell planes.
-inline uint64_t plane_stride(uint64_t n) { return ((n + 63) / 64) * 64 + 64; }
+// Population layout in HBM: row-blocked.  A row is one x-line (iy, iz) of
+// the cavity, row = iz*LY + iy.  Each row holds the 27 populations of f and
+// then the 27 of g, one x-line each:
+//
+//   f_ip(ix, row) at buf + row*rs + ip*ps + ix
+//   g_ip(ix, row) at buf + row*rs + (27 + ip)*ps + ix,   rs = 54*ps,
+//
+// with ps = LX rounded up to 32 elements.  All 54 streams that a block of
+// one row reads (and writes) then lie within one 54*ps region, so their HBM
+// channels follow from the low address bits; with one plane per population
+// ([ip][z][y][x], the reference's order) the concurrent streams lie in 54
+// distinct physical pages and their channel spread depended on where the
+// driver placed each page: measured on MI355X, the same kernel ran between
+// 19.2 and 21 ms (FP64 512^3) from one allocation to the next, against
+// 18.5-18.8 ms in this layout.  Host copies convert to the reference order.
+inline uint64_t pop_stride(int lx) { return (static_cast<uint64_t>(lx) + 31) / 32 * 32; }
+inline uint64_t row_stride(int lx) { return 54 * pop_stride(lx); }
+// Elements of one buffer holding f and g of a cavity.
+inline uint64_t buffer_elems(int lx, int ly, int lz) {
+    return row_stride(lx) * static_cast<uint64_t>(ly) * static_cast<uint64_t>(lz);
+}
 
 inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
     int64_t ntx = (lx + bx - 1) / bx;
@@ -130,18 +147,21 @@ enum AaLayout { AA_LAYOUT_N = 0, AA_LAYOUT_E = 1 };
 
 }  // namespace lbm
 
-// One cavity's device state (opaque to C callers).  Buffers f[2], g[2] hold
-// 27 planes each (stride ps); in REF phase f[cf], g[cg] are the reference's
-// post-streaming populations, in FUSED phase its post-collision ones.  With
-// the in-place pattern only f[0], g[0] exist (cf = cg = 0) and aa_layout
-// tells which of the two layouts they hold (always N in REF phase).
+// One cavity's device state (opaque to C callers).  Each buffer buf[k]
+// holds f and g (row-blocked, see pop_stride); f[k] and g[k] point at their
+// first rows.  In REF phase f[cf], g[cg] are the reference's post-streaming
+// populations, in FUSED phase its post-collision ones.  With the in-place
+// pattern only buf[0] exists (cf = cg = 0) and aa_layout tells which of the
+// two layouts it holds (always N in REF phase).
 struct lbm_ctx {
     lbm_params p{};
     lbm_derived d{};
     double beta = 0.0;
     uint64_t n = 0;
-    uint64_t ps = 0;  // population plane stride (elements)
+    uint64_t ps = 0;  // population stride (elements)
+    uint64_t rs = 0;  // row stride (elements)
     int prec = 0;
+    void *buf[2] = {nullptr, nullptr};
     void *f[2] = {nullptr, nullptr};
     void *g[2] = {nullptr, nullptr};
     int cf = 0, cg = 0;

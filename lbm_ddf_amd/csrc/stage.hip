@@ -22,6 +22,13 @@ __device__ __forceinline__ void cell_coords(uint64_t idx, int lx, int ly, int &i
 
 inline dim3 flat_grid(uint64_t n) { return dim3(static_cast<uint32_t>((n + kFlatBlock - 1) / kFlatBlock)); }
 
+// Slot of population 0 of the cell with flat index idx (x fastest) in the
+// row-blocked population layout; population ip is ip*ps further.
+__device__ __forceinline__ uint64_t slot_of(uint64_t idx, int lx, uint64_t rs) {
+    const uint64_t row = idx / static_cast<uint64_t>(lx);
+    return row * rs + (idx - row * lx);
+}
+
 // Counter-based draw (splitmix64 finaliser); matches the oracle's
 // orc_perturbation bit for bit.
 __device__ __forceinline__ double perturbation(uint64_t seed, uint64_t idx, double amp) {
@@ -48,7 +55,8 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
         const double t = a.t_hot - (a.t_hot - a.t_cold) * iy / (a.ly - 1.0);
         phi = t + perturbation(seed, idx, amp);
     }
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;
+    const uint64_t at = slot_of(idx, a.lx, a.rs);
     a.rho[idx] = 0.0;
     a.ux[idx] = 0.0;
     a.uy[idx] = 0.0;
@@ -65,8 +73,8 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
         const double feq = weight(i) * rho * (1.0 - 1.5 * usq);
         const double fv = feq - 0.5 * fy * CY[i] / (1.0 / 3.0) * feq;
         const double geq = weight(i) * phi * (1.0 - 1.5 * usq);
-        f[i * n + idx] = P::enc_f(static_cast<typename P::T>(fv));
-        g[i * n + idx] = P::template enc_g<i>(static_cast<typename P::T>(geq), tref);
+        f[at + i * ps] = P::enc_f(static_cast<typename P::T>(fv));
+        g[at + i * ps] = P::template enc_g<i>(static_cast<typename P::T>(geq), tref);
     });
 }
 
@@ -81,7 +89,8 @@ __global__ void forcing_kernel(StepArgs a) {
 __global__ void collide_f_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;
+    const uint64_t at = slot_of(idx, a.lx, a.rs);
     double *f = static_cast<double *>(a.fb);
     const double fy9 = a.fy[idx], rho9 = a.rho[idx];
     const double u9 = a.ux[idx], v9 = a.uy[idx], w9 = a.uz[idx];
@@ -92,10 +101,10 @@ __global__ void collide_f_kernel(StepArgs a) {
         const double eu = (CX[ip] * u9 + CY[ip] * v9 + CZ[ip] * w9) / RT;
         const double uv = (u9 * u9 + v9 * v9 + w9 * w9) / RT;
         const double feq = weight(ip) * (rho9 + eu + 0.5 * (eu * eu - uv));
-        double fchange = -(f[ip * n + idx] - feq) / tau;
+        double fchange = -(f[at + ip * ps] - feq) / tau;
         const double feq_rho0 = weight(ip) * (1.0 + eu + 0.5 * (eu * eu - uv));
         fchange += (1.0 - 0.5 / tau) * fy9 * (CY[ip] - v9) / RT * feq_rho0;
-        f[ip * n + idx] += fchange;
+        f[at + ip * ps] += fchange;
     });
 }
 
@@ -103,7 +112,8 @@ __global__ void collide_f_kernel(StepArgs a) {
 __global__ void collide_g_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;
+    const uint64_t at = slot_of(idx, a.lx, a.rs);
     double *g = static_cast<double *>(a.gb);
     const double phi9 = a.phi[idx];
     const double u9 = a.ux[idx], v9 = a.uy[idx], w9 = a.uz[idx];
@@ -111,12 +121,12 @@ __global__ void collide_g_kernel(StepArgs a) {
     const double tauc = a.tauc;
     static_for<Q>([&](auto I) {
         constexpr int ip = I;
-        const double g9 = g[ip * n + idx];
+        const double g9 = g[at + ip * ps];
         const double eu = (CX[ip] * u9 + CY[ip] * v9 + CZ[ip] * w9) / RT;
         const double uv = (u9 * u9 + v9 * v9 + w9 * w9) / RT;
         const double geq = weight(ip) * phi9 * (1.0 + eu + 0.5 * (eu * eu - uv));
         const double gchange = -(g9 - geq) / tauc;
-        g[ip * n + idx] += gchange;
+        g[at + ip * ps] += gchange;
     });
 }
 
@@ -129,22 +139,23 @@ __global__ void stream_kernel(StepArgs a) {
     int ix, iy, iz;
     cell_coords(idx, a.lx, a.ly, ix, iy, iz);
     const int lx = a.lx, ly = a.ly, lz = a.lz;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;
+    const uint64_t at = slot_of(idx, lx, a.rs);
     const double *src = static_cast<const double *>(SCALAR ? a.ga : a.fa);
     double *dst = static_cast<double *>(SCALAR ? a.gb : a.fb);
     static_for<Q>([&](auto I) {
         constexpr int ip = I;
         const int sx = ix - CX[ip], sy = iy - CY[ip], sz = iz - CZ[ip];
         if (sx >= 0 && sx < lx && sy >= 0 && sy < ly && sz >= 0 && sz < lz) {
-            const uint64_t sidx = (static_cast<uint64_t>(sz) * ly + sy) * lx + sx;
-            dst[ip * n + idx] = src[ip * n + sidx];
+            const uint64_t sat = (static_cast<uint64_t>(sz) * ly + sy) * a.rs + sx;
+            dst[at + ip * ps] = src[sat + ip * ps];
         }
     });
     // Wall blocks: each writes the directions pointing into the domain from
     // that wall.  x walls, y walls, z walls, in that order.
     auto bb = [&](auto I) {
         constexpr int ip = I;
-        dst[ip * n + idx] = src[OPP[ip] * n + idx];
+        dst[at + ip * ps] = src[at + OPP[ip] * ps];
     };
     static_for<Q>([&](auto I) {
         constexpr int ip = I;
@@ -159,12 +170,12 @@ __global__ void stream_kernel(StepArgs a) {
         static_for<Q>([&](auto I) {
             constexpr int ip = I;
             if constexpr (CY[ip] > 0)
-                if (iy == 0) dst[ip * n + idx] = -src[OPP[ip] * n + idx] + 2.0 * weight(ip) * a.t_hot * (1.0 + density);
+                if (iy == 0) dst[at + ip * ps] = -src[at + OPP[ip] * ps] + 2.0 * weight(ip) * a.t_hot * (1.0 + density);
         });
         static_for<Q>([&](auto I) {
             constexpr int ip = I;
             if constexpr (CY[ip] < 0)
-                if (iy == ly - 1) dst[ip * n + idx] = -src[OPP[ip] * n + idx] + 2.0 * weight(ip) * a.t_cold * (1.0 + density);
+                if (iy == ly - 1) dst[at + ip * ps] = -src[at + OPP[ip] * ps] + 2.0 * weight(ip) * a.t_cold * (1.0 + density);
         });
     } else {
         static_for<Q>([&](auto I) {
@@ -190,14 +201,15 @@ __global__ void stream_kernel(StepArgs a) {
 __global__ void macrovar_kernel(StepArgs a) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;
+    const uint64_t at = slot_of(idx, a.lx, a.rs);
     const double *f = static_cast<const double *>(a.fa);
     const double *g = static_cast<const double *>(a.ga);
     double rho = 0.0, ux = 0.0, uy = 0.0, uz = 0.0, phi = 0.0;
     static_for<Q>([&](auto I) {
         constexpr int ip = I;
-        const double fv = f[ip * n + idx];
-        const double gv = g[ip * n + idx];
+        const double fv = f[at + ip * ps];
+        const double gv = g[at + ip * ps];
         rho += fv;
         ux += CX[ip] * fv;
         uy += CY[ip] * fv;
@@ -300,7 +312,7 @@ template <class P, int WHICH, int IP>
 __global__ void decode_kernel(const void *src, double *dst, uint64_t n, CodecArgs c) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const auto s = static_cast<const typename P::S *>(src)[i];
+    const auto s = static_cast<const typename P::S *>(src)[slot_of(i, c.lx, c.rs)];
     if constexpr (WHICH == 0) dst[i] = static_cast<double>(P::dec_f(s));
     else dst[i] = static_cast<double>(P::template dec_g<IP>(s, static_cast<typename P::T>(codec_tref(i, c))));
 }
@@ -311,8 +323,9 @@ __global__ void encode_kernel(const double *src, void *dst, uint64_t n, CodecArg
     if (i >= n) return;
     const auto v = static_cast<typename P::T>(src[i]);
     auto *d = static_cast<typename P::S *>(dst);
-    if constexpr (WHICH == 0) d[i] = P::enc_f(v);
-    else d[i] = P::template enc_g<IP>(v, static_cast<typename P::T>(codec_tref(i, c)));
+    const uint64_t at = slot_of(i, c.lx, c.rs);
+    if constexpr (WHICH == 0) d[at] = P::enc_f(v);
+    else d[at] = P::template enc_g<IP>(v, static_cast<typename P::T>(codec_tref(i, c)));
 }
 
 template <class P>

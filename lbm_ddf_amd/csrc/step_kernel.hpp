@@ -350,8 +350,10 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     using T = typename P::T;
     using L = typename P::L;
     const int lx = a.lx, ly = a.ly, lz = a.lz;
-    const uint64_t n = a.ps;  // population plane stride
+    const uint64_t ps = a.ps;  // population stride within a row
     const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
+    // this cell's slot of population 0 (row-blocked layout, see pop_stride)
+    const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
     const S *__restrict__ fa = static_cast<const S *>(a.fa);
     const S *__restrict__ ga = static_cast<const S *>(a.ga);
     S *__restrict__ fb = static_cast<S *>(a.fb);
@@ -371,8 +373,8 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     if constexpr (MODE == MODE_PROLOGUE) {
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            f[i] = P::dec_f(ld<NTL>(&fa[i * n + idx]));
-            g[i] = P::template dec_g<i>(ld<NTL>(&ga[i * n + idx]), tref_y);
+            f[i] = P::dec_f(ld<NTL>(&fa[at + i * ps]));
+            g[i] = P::template dec_g<i>(ld<NTL>(&ga[at + i * ps]), tref_y);
         });
         rho = static_cast<T>(a.rho[idx]);
         ux = static_cast<T>(a.ux[idx]);
@@ -384,7 +386,6 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         const bool xlo = ix > 0, xhi = ix < lx - 1;
         const bool ylo = iy > 0, yhi = iy < ly - 1;
         const bool zlo = iz > 0, zhi = iz < lz - 1;
-        const int sy = lx, sz = lx * ly;
         const bool wall = wall_lo || wall_hi;
         L rw_raw = L(0);
         if (wall) rw_raw = wall_rho[wall_idx];
@@ -398,16 +399,18 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             constexpr int i = I;
             constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
             if constexpr (i == 0) {
-                fr[0] = ld<NTL>(&fa[idx]);
-                gr[0] = ld<NTL>(&ga[idx]);
+                fr[0] = ld<NTL>(&fa[at]);
+                gr[0] = ld<NTL>(&ga[at]);
                 pulled[0] = true;
             } else {
                 const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                 const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
                 const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
-                const uint32_t src = idx - static_cast<uint32_t>(cx + cy * sy + cz * sz);
+                // slot of population i at the source cell x - c_i
+                const uint64_t src = at + i * ps - static_cast<uint64_t>(static_cast<int64_t>(cx) +
+                                                                         static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs));
                 pulled[i] = inx && iny && inz;
-                const uint64_t off = pulled[i] ? i * n + src : o * n + idx;
+                const uint64_t off = pulled[i] ? src : at + o * ps;
                 fr[i] = ld<NTL && cx == 0>(&fa[off]);
                 gr[i] = ld<NTL && cx == 0>(&ga[off]);
             }
@@ -462,8 +465,8 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         if constexpr (MODE == MODE_FINALIZE) {
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
-                st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-                st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], tref_y));
+                st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
+                st<NTS>(&gb[at + i * ps], P::template enc_g<i>(g[i], tref_y));
             });
             a.rho[idx] = static_cast<double>(rho);
             a.ux[idx] = static_cast<double>(ux);
@@ -480,8 +483,8 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            st<NTS>(&fb[i * n + idx], P::enc_f(f[i]));
-            st<NTS>(&gb[i * n + idx], P::template enc_g<i>(g[i], tref_y));
+            st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
+            st<NTS>(&gb[at + i * ps], P::template enc_g<i>(g[i], tref_y));
         });
         lag[idx] = static_cast<L>(phi);
         if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);

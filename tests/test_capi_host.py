@@ -62,8 +62,7 @@ def test_bytes_required_in_place_pattern():
     ab = capi.bytes_required(capi.make_params(512, 512, 512))
     aa = capi.bytes_required(capi.make_params(512, 512, 512, pattern="aa"))
     n = 512 ** 3
-    ps = ((n + 63) // 64) * 64 + 64
-    assert ab - aa == 2 * 27 * ps * 8 + 2 * 512 * 512 * 8  # second buffers and the wall-rho array
+    assert ab - aa == 54 * n * 8 + 2 * 512 * 512 * 8  # the second f+g buffer and the wall-rho array
     assert aa < 66e9
 
 
