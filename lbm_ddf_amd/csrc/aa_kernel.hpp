@@ -3,7 +3,7 @@
 //
 // The pull A/B scheme keeps two copies of f and g.  Here one copy holds them,
 // and two layouts of it alternate.  Slot (i, x) is population i of cell x
-// (row-blocked layout, see pop_stride).
+// (see pop_layout).
 //
 //   N ("natural"):   slot (i, x) holds f_i(x) after streaming -- what the
 //                    reference stores after src/main.cu:41.
@@ -47,7 +47,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     const int lx = a.lx, ly = a.ly, lz = a.lz;
     const uint64_t ps = a.ps;  // population stride within a row
     const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
-    // this cell's slot of population 0 (row-blocked layout, see pop_stride)
+    // this cell's slot of population 0 (see pop_layout)
     const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
     // one buffer per distribution, read and written in place (no __restrict__)
     S *fs = static_cast<S *>(a.fb);

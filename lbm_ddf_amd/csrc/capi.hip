@@ -257,7 +257,7 @@ bool valid_flags(bool aa, int flags) {
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     const bool aa = p->pattern == LBM_PATTERN_AA;
-    const uint64_t pops = (aa ? 1ull : 2ull) * lbm::buffer_elems(p->lx, p->ly, p->lz) * elem_size(p->precision);
+    const uint64_t pops = (aa ? 1ull : 2ull) * lbm::pop_layout(p->lx, p->ly, p->lz).elems * elem_size(p->precision);
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + (aa ? 0ull : 2ull * p->lx * p->lz * side_size(p->precision));
     (void)bx;
@@ -341,8 +341,9 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
     c->prec = p->precision;
     c->aa = p->pattern == LBM_PATTERN_AA;
     c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
-    c->ps = lbm::pop_stride(p->lx);
-    c->rs = lbm::row_stride(p->lx);
+    const lbm::PopLayout lay = lbm::pop_layout(p->lx, p->ly, p->lz);
+    c->ps = lay.ps;
+    c->rs = lay.rs;
     derive(p, &c->d, &c->beta);
     c->bx = default_block(p->lx);
     hipError_t e = hipGetDevice(&c->device);
@@ -363,7 +364,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
         return hip_fail(e, "stream/event creation");
     }
     const uint64_t n = c->n;
-    const size_t buf_bytes = lbm::buffer_elems(p->lx, p->ly, p->lz) * elem_size(c->prec);
+    const size_t buf_bytes = lay.elems * elem_size(c->prec);
     struct Alloc {
         void **ptr;
         size_t bytes;
@@ -390,7 +391,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
             return fail(LBM_E_NOMEM, "device allocation of " + std::to_string(al.bytes) + " bytes failed");
         }
     }
-    // f and g of one buffer share its rows: g starts 27 populations later
+    // g starts 27 populations after f (within each row, or after f's planes)
     for (int k = 0; k < 2; ++k) {
         if (!c->buf[k]) continue;
         c->f[k] = c->buf[k];

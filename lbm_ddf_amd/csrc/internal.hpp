@@ -115,26 +115,35 @@ hipError_t launch_decode_plane(int precision, int which, int ip, const void *src
 hipError_t launch_encode_plane(int precision, int which, int ip, const double *src, void *dst,
                                uint64_t n, const CodecArgs &c, hipStream_t s);
 
-// Population layout in HBM: row-blocked.  A row is one x-line (iy, iz) of
-// the cavity, row = iz*LY + iy.  Each row holds the 27 populations of f and
-// then the 27 of g, one x-line each:
+// Population layout in HBM.  Element (population ip, cell (ix, row)) of f
+// lies at buf + row*rs + ip*ps + ix and of g at 27*ps further, with
+// row = iz*LY + iy the cell's x-line.  Two layouts share this formula:
 //
-//   f_ip(ix, row) at buf + row*rs + ip*ps + ix
-//   g_ip(ix, row) at buf + row*rs + (27 + ip)*ps + ix,   rs = 54*ps,
+//   row-blocked (rs = 54*LX, ps = LX): each x-line holds its 27 f and 27 g
+//     populations side by side, so the 54 streams that a block of one line
+//     reads and writes lie within 54*LX elements and their HBM channels
+//     follow from the low address bits.  With one plane per population the
+//     concurrent streams lie in 54 distinct physical pages and their channel
+//     spread depended on where the driver placed each page: measured on
+//     MI355X, the same FP64 512^3 kernel ran between 19.2 and 21 ms from one
+//     allocation to the next, against 18.6-19.4 ms row-blocked.
+//   planes (rs = LX, ps >= N): the reference's [ip][z][y][x] with each plane
+//     staggered by 64 elements.  Used unless LX is a multiple of kRowsLx:
+//     ensemble tiles (up to 256 cells of consecutive x-lines) and the last
+//     block of a line then cover cells of two x-lines, which are contiguous
+//     only in this layout (measured: 48 cubes of side 33-256, FP16, 19.9k
+//     MLUPS in planes against 13.7k row-blocked).
 //
-// with ps = LX rounded up to 32 elements.  All 54 streams that a block of
-// one row reads (and writes) then lie within one 54*ps region, so their HBM
-// channels follow from the low address bits; with one plane per population
-// ([ip][z][y][x], the reference's order) the concurrent streams lie in 54
-// distinct physical pages and their channel spread depended on where the
-// driver placed each page: measured on MI355X, the same kernel ran between
-// 19.2 and 21 ms (FP64 512^3) from one allocation to the next, against
-// 18.5-18.8 ms in this layout.  Host copies convert to the reference order.
-inline uint64_t pop_stride(int lx) { return (static_cast<uint64_t>(lx) + 31) / 32 * 32; }
-inline uint64_t row_stride(int lx) { return 54 * pop_stride(lx); }
-// Elements of one buffer holding f and g of a cavity.
-inline uint64_t buffer_elems(int lx, int ly, int lz) {
-    return row_stride(lx) * static_cast<uint64_t>(ly) * static_cast<uint64_t>(lz);
+// Host copies convert to the reference's order.
+constexpr int kRowsLx = 256;
+struct PopLayout {
+    uint64_t ps, rs, elems;  // population stride, row stride, elements of one f+g buffer
+};
+inline PopLayout pop_layout(int lx, int ly, int lz) {
+    const uint64_t n = static_cast<uint64_t>(lx) * ly * lz;
+    if (lx % kRowsLx == 0) return {static_cast<uint64_t>(lx), 54ull * lx, 54ull * n};
+    const uint64_t ps = (n + 63) / 64 * 64 + 64;
+    return {ps, static_cast<uint64_t>(lx), 54ull * ps};
 }
 
 inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
@@ -148,8 +157,8 @@ enum AaLayout { AA_LAYOUT_N = 0, AA_LAYOUT_E = 1 };
 }  // namespace lbm
 
 // One cavity's device state (opaque to C callers).  Each buffer buf[k]
-// holds f and g (row-blocked, see pop_stride); f[k] and g[k] point at their
-// first rows.  In REF phase f[cf], g[cg] are the reference's post-streaming
+// holds f and g (see pop_layout); f[k] and g[k] point at their first
+// elements.  In REF phase f[cf], g[cg] are the reference's post-streaming
 // populations, in FUSED phase its post-collision ones.  With the in-place
 // pattern only buf[0] exists (cf = cg = 0) and aa_layout tells which of the
 // two layouts it holds (always N in REF phase).

@@ -352,7 +352,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     const int lx = a.lx, ly = a.ly, lz = a.lz;
     const uint64_t ps = a.ps;  // population stride within a row
     const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
-    // this cell's slot of population 0 (row-blocked layout, see pop_stride)
+    // this cell's slot of population 0 (see pop_layout)
     const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
     const S *__restrict__ fa = static_cast<const S *>(a.fa);
     const S *__restrict__ ga = static_cast<const S *>(a.ga);

@@ -31,7 +31,7 @@ def make_pair(lx, ly, lz, seed=1, amp=1.0e-3, init="conductive", **kw):
 
 
 @pytest.mark.parametrize("nsteps", [1, 2, 3, 8, 23])
-@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (1, 4, 3), (3, 3, 1), (130, 5, 3)])
+@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (1, 4, 3), (3, 3, 1), (130, 5, 3), (256, 5, 3)])
 def test_aa_fused_bitwise(shape, nsteps):
     c, o = make_pair(*shape, seed=3, amp=5.0e-2, rayleigh=1.0e6)
     c.step(nsteps)
@@ -55,7 +55,7 @@ def test_aa_split_calls_reads_and_stages():
     assert_state_equal(c, o)
 
 
-@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 4, 3), (3, 3, 1), (70, 6, 5)])
+@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 4, 3), (3, 3, 1), (70, 6, 5), (256, 4, 3)])
 def test_aa_each_stage_bitwise(shape):
     """The stage interface in place (flip + exchange streaming) against the
     oracle's stages."""
@@ -149,13 +149,14 @@ def test_aa_pops_roundtrip_and_restart(tmp_path):
         np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (256, 24, 8)])
 @pytest.mark.parametrize("prec,tol", [("fp16", 2e-3), ("bf16", 2e-2)])
-def test_aa_reduced_precision_tracks_fp64(prec, tol):
-    """16-bit storage in place follows the FP64 run (32^3, Ra=1e5, 300 and 301
-    steps) as closely as the A/B 16-bit path does."""
-    ref = Cavity(32, 32, 32, **RB)
-    lp = Cavity(32, 32, 32, precision=prec, pattern="aa", **RB)
-    lp_ab = Cavity(32, 32, 32, precision=prec, **RB)
+def test_aa_reduced_precision_tracks_fp64(prec, tol, shape):
+    """16-bit storage in place follows the FP64 run (Ra=1e5, 300 and 301
+    steps; both population layouts) as closely as the A/B 16-bit path does."""
+    ref = Cavity(*shape, **RB)
+    lp = Cavity(*shape, precision=prec, pattern="aa", **RB)
+    lp_ab = Cavity(*shape, precision=prec, **RB)
     for c in (ref, lp, lp_ab):
         c.init_conductive(1, 1.0e-3)
     for n in (300, 1):

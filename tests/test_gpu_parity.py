@@ -63,7 +63,9 @@ def test_initial_state_bitwise(shape, init):
     assert_state_equal(c, o)
 
 
-@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 4, 3), (3, 3, 1), (70, 6, 5)])
+# (256, 4, 3) and (256, 5, 3): x-lines of 256 cells select the row-blocked
+# population layout; the other shapes the plane layout.
+@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 4, 3), (3, 3, 1), (70, 6, 5), (256, 4, 3)])
 def test_each_stage_bitwise(shape):
     """Every stage entry point against the oracle stage, two full steps."""
     c, o = make_pair(*shape, seed=7, amp=1.0e-2)
@@ -76,7 +78,7 @@ def test_each_stage_bitwise(shape):
 
 
 @pytest.mark.parametrize("nsteps", [1, 2, 5, 23])
-@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (130, 5, 3)])
+@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (130, 5, 3), (256, 5, 3)])
 def test_fused_step_bitwise(shape, nsteps):
     c, o = make_pair(*shape, seed=3, amp=5.0e-2, rayleigh=1.0e6)
     c.step(nsteps)
@@ -173,19 +175,26 @@ def test_profile_matches_oracle():
     np.testing.assert_allclose(g[:, 4], r[:, 4], rtol=0, atol=1e-12 * np.max(np.abs(r[:, 3])) / np.min(r[:, 4]))
 
 
-def test_pops_roundtrip_fp64():
-    c, _ = make_pair(7, 5, 6)
-    f = np.random.default_rng(0).standard_normal((27, 6, 5, 7))
+@pytest.mark.parametrize("shape", [(7, 5, 6), (256, 3, 2)])
+def test_pops_roundtrip_fp64(shape):
+    c, _ = make_pair(*shape)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal((27,) + shape[::-1])
+    g = rng.standard_normal((27,) + shape[::-1])
     c.set_pops("f", f)
+    c.set_pops("g", g)
     np.testing.assert_array_equal(c.get_pops("f"), f)
+    np.testing.assert_array_equal(c.get_pops("g"), g)
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (256, 24, 8)])
 @pytest.mark.parametrize("prec,tol", [("fp16", 2e-3), ("bf16", 2e-2)])
-def test_reduced_precision_tracks_fp64(prec, tol):
-    """16-bit storage with FP32 arithmetic follows the FP64 run (32^3,
-    Ra=1e5, 300 steps): max |dT| below tol, mass drift small, no NaN."""
-    ref = Cavity(32, 32, 32, **RB)
-    lp = Cavity(32, 32, 32, precision=prec, **RB)
+def test_reduced_precision_tracks_fp64(prec, tol, shape):
+    """16-bit storage with FP32 arithmetic follows the FP64 run (Ra=1e5,
+    300 steps; both population layouts): max |dT| below tol, mass drift
+    small, no NaN."""
+    ref = Cavity(*shape, **RB)
+    lp = Cavity(*shape, precision=prec, **RB)
     ref.init_conductive(1, 1.0e-3)
     lp.init_conductive(1, 1.0e-3)
     ref.step(300)
