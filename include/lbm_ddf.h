@@ -254,7 +254,8 @@ int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_pa
 #define LBM_REQ_CONVERGED 0  /* stopping rule met */
 #define LBM_REQ_BUDGET 1     /* max_steps reached */
 #define LBM_REQ_REJECTED 2   /* alone larger than the HBM budget */
-#define LBM_REQ_DIVERGED 3   /* non-finite moments: retired, others continue */
+#define LBM_REQ_DIVERGED 3   /* non-finite moment or max|u| above the sound speed
+                                1/sqrt(3): retired, others continue */
 #define LBM_REQ_UNFINISHED 4 /* admitted, time limit reached first */
 #define LBM_REQ_PENDING 5    /* never admitted before the time limit */
 

@@ -13,6 +13,8 @@
 // Nu_bulk (src/output.cu:228-233); the running Nu is the mean of its samples;
 // the request is converged when the running Nu changes by less than nu_tol
 // between two checks (at least three samples), or stops at its step budget.
+// A cavity with a non-finite moment or a speed above the lattice sound speed
+// is retired as diverged; the others continue.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -24,6 +26,9 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Lattice sound speed of D3Q27 (c_s^2 = 1/3).
+const double kSoundSpeed = std::sqrt(1.0 / 3.0);
 
 struct Live {
     int32_t req = -1;
@@ -182,7 +187,9 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
                 const int64_t done = d.step;  // the cavity's completed step at the chunk end
                 cell_updates += static_cast<double>(l.cells) * static_cast<double>(c->nstep - before[i]);
                 const lbm_request &q = reqs[l.req];
-                if (d.nonfinite != 0) {
+                // health: a non-finite moment, or a speed beyond the lattice
+                // sound speed 1/sqrt(3), where the BGK model has left its range
+                if (d.nonfinite != 0 || !(d.max_speed <= kSoundSpeed)) {
                     finish(l, LBM_REQ_DIVERGED, &d, done);
                     continue;
                 }

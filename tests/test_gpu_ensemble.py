@@ -134,3 +134,23 @@ def test_autotune_and_cache(tmp_path):
         assert_member_equals(e, s, cav)
     e.close()
     e2.close()
+
+
+def test_injected_nan_stays_in_its_cavity():
+    """Host-forced fault: a NaN written into one member's populations shows up
+    in that member's health record only; the other members still equal
+    their standalone runs bit for bit."""
+    e, slots = build()
+    e.step(5)
+    bad = e.cavity(slots[1])
+    f = bad.get_pops("f")
+    f[13, 1, 2, 3] = np.nan
+    bad.set_pops("f", f)
+    diags = e.run_diag(6)
+    assert diags[slots[1]]["nonfinite"] > 0
+    for s, cav in zip(slots, CAVS):
+        if s == slots[1]:
+            continue
+        assert diags[s]["nonfinite"] == 0
+        assert_member_equals(e, s, cav)
+    e.close()

@@ -11,7 +11,7 @@ import gc
 import numpy as np
 import pytest
 
-from lbm_ddf_amd import Cavity
+from lbm_ddf_amd import Cavity, LbmError, bytes_required, make_params
 
 pytestmark = pytest.mark.gpu
 
@@ -60,6 +60,28 @@ def test_512_fused_equals_staged_fp64():
         c.init_conductive(4, 1.0e-3)
         c.step_staged(2)
         _same(_snapshot(c), fused)
+    finally:
+        c.close()
+        gc.collect()
+
+
+def test_largest_cavity_in_place():
+    """768^3 FP64 (453 M cells): two population buffers would need about
+    417 GB and creation fails cleanly (out of memory, nothing left
+    allocated); one buffer updated in place needs about 221 GB and runs."""
+    big = 768
+    assert bytes_required(make_params(big, big, big)) > 400e9
+    assert bytes_required(make_params(big, big, big, pattern="aa")) < 230e9
+    with pytest.raises(LbmError, match="lbm error -3"):
+        Cavity(big, big, big, **RB)
+    c = Cavity(big, big, big, pattern="aa", **RB)
+    try:
+        c.init_conductive(5, 1.0e-3)
+        c.step(3)
+        d = c.reduce()
+        assert d["step"] == 3 and d["nonfinite"] == 0
+        assert abs(d["mass"]) < 1e-6
+        assert d["max_speed"] < 1e-2
     finally:
         c.close()
         gc.collect()

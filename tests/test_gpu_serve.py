@@ -73,7 +73,7 @@ def test_resident_cells_cap():
 
 def test_reject_divergence_and_time_limit():
     big = req(64, 64, 64, 1e5, 9, max_steps=100)
-    blowup = req(12, 12, 12, 1e5, 10, amp=1e4, max_steps=400)  # absurd perturbation: non-finite quickly
+    blowup = req(12, 12, 12, 1e5, 10, amp=1e4, max_steps=400)  # absurd perturbation: supersonic, then non-finite
     ok = req(12, 12, 12, 1e5, 11, max_steps=200)
     need_small = bytes_required(make_params(12, 12, 12))
     res, st = serve([big, blowup, ok], chunk_steps=50, hbm_budget=3 * need_small)

@@ -17,8 +17,10 @@ textbook solutions, each chosen so that a plausible transcription mistake
                   profile puts the walls half a cell outside the end nodes
   f subsystem     hydrostatic balance under a uniform force: u -> 0 and
                   d rho'/dy = 3 F
-  full step       rest state is a fixed point; mass is conserved; x mirror
-                  symmetry; conduction below onset gives Nu_bulk -> 1
+  full step       rest state is a fixed point; mass is conserved; x and z
+                  mirror symmetry; the x <-> z swap symmetry is broken at the
+                  wall edges by the wall order; conduction below onset gives
+                  Nu_bulk -> 1
   output_nu       Nu formulas on fields with known answers
 """
 import itertools
@@ -261,6 +263,43 @@ def test_x_mirror_symmetry():
     s.run(40)
     np.testing.assert_allclose(s["phi"], s["phi"][:, :, ::-1], rtol=0, atol=1e-13)
     np.testing.assert_allclose(s["ux"], -s["ux"][:, :, ::-1], rtol=0, atol=1e-15)
+
+
+def test_z_mirror_symmetry():
+    LZ = 8
+    s = oracle.OracleState(6, 10, LZ, rayleigh=1e6, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    s.init_conductive(4, 1e-2)
+    phi = s["phi"]
+    sym = 0.5 * (phi + phi[::-1, :, :])
+    s["phi"] = sym
+    s["g"] = W[:, None, None, None] * sym
+    s.forcing()
+    s.run(40)
+    np.testing.assert_allclose(s["phi"], s["phi"][::-1, :, :], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(s["uz"], -s["uz"][::-1, :, :], rtol=0, atol=1e-15)
+
+
+def test_x_z_swap_asymmetry_at_the_wall_edges():
+    """The reference applies the wall rules x, then y, then z
+    (src/streaming.cu:127-251): a g population whose source lies beyond a y
+    wall and a z wall gets the adiabatic bounce-back, one beyond a y wall and
+    an x wall the isothermal anti-bounce-back.  A state symmetric under the
+    x <-> z swap therefore loses that symmetry, first on the edges where the
+    y walls meet the z walls; the interior away from the edges keeps it."""
+    L = 8
+    s = oracle.OracleState(L, 6, L, rayleigh=1e4, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    s.init_conductive(2, 0.0)  # laterally uniform: symmetric under x <-> z
+    s.run(3)
+    phi = s["phi"]  # [z][y][x]
+    swapped = np.transpose(phi, (2, 1, 0))
+    diff = np.abs(phi - swapped)
+    assert diff.max() > 1e-6
+    # after 3 steps the asymmetry has spread at most 3 cells from a wall edge
+    interior = diff[3:L - 3, :, 3:L - 3]
+    assert interior.max() < 1e-14
+    # the edges where the y walls meet the z walls differ from their x
+    # counterparts
+    assert np.abs(phi[0, 0, 3:L - 3] - phi[3:L - 3, 0, 0]).max() > 1e-6
 
 
 def test_conduction_below_onset_gives_unit_bulk_nusselt():
