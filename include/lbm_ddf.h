@@ -19,9 +19,12 @@
  *   - host buffers are caller-owned; device memory is library-owned;
  *   - a context is not thread-safe; distinct contexts are independent.
  *
- * Populations are stored structure-of-arrays [ip][iz][iy][ix] (x fastest),
- * the reference layout (src/collision.cu:10, src/streaming.cu:9), with
- * 64-bit offsets (the reference's int offsets overflow at 512^3).
+ * Host copies of populations (lbm_get_pops / lbm_set_pops, checkpoints) use
+ * the reference layout [ip][iz][iy][ix], x fastest (src/collision.cu:10,
+ * src/streaming.cu:9).  On the device the library chooses per cavity between
+ * that layout (planes staggered in HBM) and a row-blocked one (the 54
+ * populations of an x-line side by side); offsets are 64-bit (the
+ * reference's int offsets overflow at 512^3).
  */
 #ifndef LBM_DDF_H
 #define LBM_DDF_H

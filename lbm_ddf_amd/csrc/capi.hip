@@ -1,8 +1,9 @@
 // capi.hip -- the C ABI of include/lbm_ddf.h: context, phases, host copies.
 //
-// A context holds two population buffers per distribution (pull A/B scheme),
-// the reference-state moment fields, and the two per-cell side values of the
-// fused kernel.  It is in one of three phases:
+// A context holds two population buffers, each with f and g (pull A/B
+// scheme; population layout: internal.hpp pop_layout), the reference-state
+// moment fields, and the two per-cell side values of the fused kernel.  It
+// is in one of three phases:
 //   EMPTY  allocated, no state yet;
 //   REF    populations are the reference's post-streaming f, g and the
 //          fields rho', u, phi, F_y are the reference's (as after
@@ -11,8 +12,8 @@
 //          step is pending), with lag phi and wall rho' valid.
 // Calls that need the reference state run the finalize pass first.
 //
-// With the in-place pattern (LBM_PATTERN_AA) there is one buffer per
-// distribution and the passes alternate between its two layouts
+// With the in-place pattern (LBM_PATTERN_AA) there is one buffer and the
+// passes alternate between its two slot layouts
 // (aa_kernel.hpp): REF phase always holds layout N; in FUSED phase aa_layout
 // says whether the next pass is odd (layout E) or even (layout N).  Finalize
 // is the moments pass, preceded by the in-place exchange when the layout is E.
@@ -634,10 +635,8 @@ int lbm_set_tuning(lbm_ctx *c, int32_t block_x, int32_t flags) {
     if (!valid_block(c->p.lx, block_x)) return fail(LBM_E_INVALID, "block_x must be 64, 128, 256 or 512 and fit LX");
     if (!valid_flags(c->aa, flags))
         return fail(LBM_E_INVALID, c->aa ? "flags must be 0, 3, 4 or 7 (in-place pattern)" : "flags must be in 0..7");
-    if (block_x != c->bx && c->phase == PHASE_FUSED) {
-        // partial-sum layouts differ per block width only for reductions,
-        // which are finished within one call; switching is safe at any time.
-    }
+    // Partial sums depend on the block width only within one reducing call,
+    // and every variant computes the same values: switching is safe at any time.
     c->bx = block_x;
     c->flags = flags;
     return LBM_OK;
