@@ -161,6 +161,46 @@ int run_pass(lbm_ctx *c, int mode, bool reduce) {
     return LBM_OK;
 }
 
+int graph_key(const lbm_ctx *c) {
+    return c->cf | c->cg << 1 | c->aa_layout << 2 | c->flags << 3 | c->bx << 6;
+}
+
+// kGraphPasses plain fused passes from the current state, replayed from a
+// graph captured once per starting state (two states alternate at most:
+// the parities after an even and after an odd number of passes).  The
+// passes launch no memset and capture their arguments by value.
+int run_fused_graph(lbm_ctx *c) {
+    const int key = graph_key(c);
+    int k = c->graph_key[0] == key ? 0 : (c->graph_key[1] == key ? 1 : -1);
+    if (k < 0) {
+        k = c->graph[0] == nullptr ? 0 : (c->graph[1] == nullptr ? 1 : 0);
+        if (c->graph[k]) {
+            LBM_HIP(hipGraphExecDestroy(c->graph[k]));
+            c->graph[k] = nullptr;
+            c->graph_key[k] = -1;
+        }
+        hipGraph_t g = nullptr;
+        LBM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = LBM_OK;
+        for (int i = 0; i < lbm::kGraphPasses && rc == LBM_OK; ++i) rc = run_pass(c, lbm::MODE_FUSED, false);
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rc != LBM_OK || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != LBM_OK ? rc : hip_fail(ec, "hipStreamEndCapture");
+        }
+        // the captured passes flipped the bookkeeping an even number of times
+        const hipError_t ei = hipGraphInstantiate(&c->graph[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->graph[k] = nullptr;
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+        c->graph_key[k] = key;
+    }
+    LBM_HIP(hipGraphLaunch(c->graph[k], c->stream));
+    return LBM_OK;
+}
+
 lbm::DiagArgs diag_args(const lbm_ctx *c, int64_t step) {
     lbm::DiagArgs d{};
     d.lx = c->p.lx;
@@ -289,6 +329,8 @@ int lbm_bytes_required(const lbm_params *p, uint64_t *bytes) {
 int lbm_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipGraphExec_t &g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     void *bufs[] = {c->buf[0], c->buf[1], c->rho, c->ux, c->uy, c->uz, c->phi, c->fy,
                     c->lag, c->wall_rho, c->partials, c->partials2, c->diag_dev, c->plane};
     for (void *b : bufs)
@@ -523,6 +565,10 @@ int lbm_step(lbm_ctx *c, int64_t nsteps) {
         c->phase = PHASE_FUSED;
         fused -= 1;
     }
+    // long runs: replay captured sequences of passes (fewer launches)
+    if (fused >= 2 * lbm::kGraphPasses)
+        for (; fused >= lbm::kGraphPasses; fused -= lbm::kGraphPasses)
+            if (int rc = run_fused_graph(c)) return rc;
     for (int64_t k = 0; k < fused; ++k)
         if (int rc = run_pass(c, lbm::MODE_FUSED, false)) return rc;
     c->nstep += nsteps;

@@ -152,6 +152,9 @@ inline int64_t step_grid_blocks(int lx, int ly, int lz, int bx) {
 }
 
 enum Phase { PHASE_EMPTY = 0, PHASE_REF = 1, PHASE_FUSED = 2 };
+// lbm_step replays runs of this many fused passes (an even number, so the
+// buffer parity returns to where it started) from a captured hipGraph.
+constexpr int kGraphPasses = 32;
 enum AaLayout { AA_LAYOUT_N = 0, AA_LAYOUT_E = 1 };
 
 }  // namespace lbm
@@ -185,6 +188,10 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false when an ensemble lends its stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Captured sequences of kGraphPasses fused passes, one per starting
+    // parity state; key = cf | cg << 1 | aa_layout << 2 | flags << 3 | bx << 6.
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_key[2] = {-1, -1};
     bool aa = false;    // in-place pattern (LBM_PATTERN_AA)
     int aa_layout = 0;  // AA_LAYOUT_N or AA_LAYOUT_E
     int phase = lbm::PHASE_EMPTY;

@@ -40,6 +40,15 @@ def test_aa_fused_bitwise(shape, nsteps):
     assert_state_equal(c, o)
 
 
+def test_aa_graph_replays_both_layouts():
+    """Replayed runs of in-place passes starting from either layout."""
+    c, o = make_pair(33, 7, 6, seed=17, amp=1.0e-2, rayleigh=1.0e6)
+    for n in (65, 1, 96, 70):
+        c.step(n)
+        o.run(n)
+        assert_state_equal(c, o)
+
+
 def test_aa_split_calls_reads_and_stages():
     """Odd and even pass parities interleaved with reads and stage calls."""
     c, o = make_pair(12, 8, 6, seed=11, amp=1.0e-2)

@@ -101,6 +101,23 @@ def test_fused_split_calls_and_reads_are_transparent():
     assert_state_equal(c2, o)
 
 
+def test_graph_replays_between_reads_and_tuning_changes():
+    """Long lbm_step calls replay captured runs of passes; the captured
+    sequence depends on the buffer parity and launch shape, so reads (which
+    complete the pending half-step), odd step counts and tuning changes
+    between calls must leave the results exact."""
+    c, o = make_pair(130, 9, 7, seed=13, amp=1.0e-2, rayleigh=1.0e6)
+    done = 0
+    for n, tuning in ((70, None), (97, (64, 3)), (64, None), (131, (256, 7)), (65, (128, 4))):
+        if tuning:
+            c.set_tuning(*tuning)
+        c.step(n)
+        done += n
+        o.run(n)
+        assert_state_equal(c, o)
+    assert c.step_count == done
+
+
 def test_fused_then_stages_then_fused():
     c, o = make_pair(9, 6, 5, seed=5, amp=1.0e-2)
     c.step(4)
