@@ -240,8 +240,9 @@ int lbm_ensemble_set_tuning(lbm_ensemble *ens, int32_t block_x, int32_t blocks_p
 int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs,
                             int32_t *flags);
 /* Times each (block_x, blocks_per_cu, flags) candidate for `steps` steps on the live
- * ensemble and keeps the fastest; with cache_path, a result stored for the
- * same precision and cavity-size multiset is reused without timing. */
+ * ensemble, times the four fastest again (twice, over 3*steps steps each) and
+ * keeps the fastest of those; with cache_path, a result stored for the same
+ * precision and cavity-size multiset is reused without timing. */
 int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_path, double *best_ms_per_step);
 
 /* ---- Serving: continuous batching of arriving solves ----------------------

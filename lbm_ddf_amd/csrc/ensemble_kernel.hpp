@@ -17,8 +17,9 @@ struct EnsCavity {
 
 template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
 __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restrict__ cav,
-                                                      const uint32_t *__restrict__ ctrl, int parity) {
-    const uint32_t ncav = ctrl[0], ntiles = ctrl[1];
+                                                      const uint32_t *__restrict__ ctrl,
+                                                      const uint32_t *__restrict__ tmap, int parity) {
+    const uint32_t ntiles = ctrl[1];
     // Tile order.  Plain: block b takes tiles b, b+G, ...  XCD-contiguous:
     // the blocks of XCD x (b % 8 == x) share the x-th eighth of the tile
     // space, so neighbouring tiles -- which share the cache lines straddled by
@@ -38,14 +39,12 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
         t_end = ntiles;
         t_step = gridDim.x;
     }
-    for (; t < t_end; t += t_step) {
-        uint32_t lo = 0, hi = ncav;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cav[mid].tile0 <= t) lo = mid;
-            else hi = mid;
-        }
-        const EnsCavity &c = cav[lo];
+    // A tile's cavity comes from the tile map, loaded one tile ahead, so
+    // that finding it costs no dependent chain of table reads.
+    uint32_t ci = t < t_end ? tmap[t] : 0u, ci_next = 0u;
+    for (; t < t_end; t += t_step, ci = ci_next) {
+        ci_next = t + t_step < t_end ? tmap[t + t_step] : 0u;
+        const EnsCavity &c = cav[ci];
         if (!c.active) continue;
         const StepArgs &a = c.a[parity];
         const uint32_t lt = t - c.tile0;
@@ -68,37 +67,39 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
 }
 
 template <class P, int BX, int FL>
-hipError_t launch_fl(const EnsCavity *cav, const uint32_t *ctrl, int parity, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX, FL>), dim3(grid), dim3(BX), 0, s, cav, ctrl, parity);
+hipError_t launch_fl(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, int grid,
+                     hipStream_t s) {
+    hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX, FL>), dim3(grid), dim3(BX), 0, s, cav, ctrl, tmap,
+                       parity);
     return hipGetLastError();
 }
 
 // Flags of the plain launch: 0, 3 (non-temporal), 4 (XCD order), 7 (both).
 // The launch that folds the diagnostics uses the XCD order only.
 template <class P, int BX>
-hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int flags, int grid,
-                     hipStream_t s) {
+hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
+                     int flags, int grid, hipStream_t s) {
     if (reduce) {
         hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX, kXcdSwizzle>), dim3(grid), dim3(BX), 0, s, cav,
-                           ctrl, parity);
+                           ctrl, tmap, parity);
         return hipGetLastError();
     }
     switch (flags) {
-        case 0: return launch_fl<P, BX, 0>(cav, ctrl, parity, grid, s);
-        case 3: return launch_fl<P, BX, 3>(cav, ctrl, parity, grid, s);
-        case 4: return launch_fl<P, BX, 4>(cav, ctrl, parity, grid, s);
-        case 7: return launch_fl<P, BX, 7>(cav, ctrl, parity, grid, s);
+        case 0: return launch_fl<P, BX, 0>(cav, ctrl, tmap, parity, grid, s);
+        case 3: return launch_fl<P, BX, 3>(cav, ctrl, tmap, parity, grid, s);
+        case 4: return launch_fl<P, BX, 4>(cav, ctrl, tmap, parity, grid, s);
+        case 7: return launch_fl<P, BX, 7>(cav, ctrl, tmap, parity, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class P>
-hipError_t launch_prec(const EnsCavity *cav, const uint32_t *ctrl, int parity, bool reduce, int bx, int flags,
-                       int grid, hipStream_t s) {
+hipError_t launch_prec(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
+                       int bx, int flags, int grid, hipStream_t s) {
     switch (bx) {
-        case 64: return launch_bx<P, 64>(cav, ctrl, parity, reduce, flags, grid, s);
-        case 128: return launch_bx<P, 128>(cav, ctrl, parity, reduce, flags, grid, s);
-        case 256: return launch_bx<P, 256>(cav, ctrl, parity, reduce, flags, grid, s);
+        case 64: return launch_bx<P, 64>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
+        case 128: return launch_bx<P, 128>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
+        case 256: return launch_bx<P, 256>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
