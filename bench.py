@@ -108,10 +108,10 @@ def run_ensemble_workload(args) -> dict:
         e.step(2)
         t = time.perf_counter()
         e.autotune(8, str(ROOT / "profiles" / "ensemble_tuning.txt") if args.tuning_cache else None)
-        bx, bpc, gp, fl = e.tuning
-        e.set_tuning(bx, bpc, 16, fl)
+        bx, bpc, gp, fl, cpt = e.tuning
+        e.set_tuning(bx, bpc, 16, fl, cpt)
         log(f"ensemble {prec}: autotune {time.perf_counter() - t:.1f} s -> block {bx}, {bpc} blocks/CU, "
-            f"flags {fl}")
+            f"flags {fl}, {cpt} cells/thread")
         e.step(32)
         e.count()
         t0 = time.perf_counter()
@@ -160,7 +160,7 @@ def run_ensemble_workload(args) -> dict:
             "speedup_vs_sequential": round(agg / seq, 2),
             "sequential_reference_structure_mlups": round(ref_seq, 1) if ref_seq else None,
             "speedup_vs_reference_structure": round(agg / ref_seq, 2) if ref_seq else None,
-            "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "graph_steps": 32},
+            "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "cells_per_thread": cpt, "graph_steps": 32},
             "nonfinite_cavities": bad,
         }
     return out

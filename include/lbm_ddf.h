@@ -232,17 +232,21 @@ int lbm_ensemble_step_timed(lbm_ensemble *ens, int64_t nsteps, double *elapsed_m
 /* nsteps launches, the last with every cavity's diagnostics folded in;
  * out[slot] receives them (step = the cavity's completed step).  cap >= slots. */
 int lbm_ensemble_run_diag(lbm_ensemble *ens, int64_t nsteps, lbm_diag *out, int32_t cap);
-/* block_x (64, 128, 256) cells per tile, blocks_per_cu of the persistent
+/* block_x (64, 128, 256) threads per block, blocks_per_cu of the persistent
  * grid, graph_pairs step pairs per graph replay (0: no graph), flags 0, 3
- * (non-temporal hints), 4 (XCD-contiguous tile order) or 7 (both). */
+ * (non-temporal hints), 4 (XCD-contiguous tile order) or 7 (both),
+ * cells_per_thread (1, 2, 4): a tile is block_x * cells_per_thread
+ * consecutive cells of one cavity.  Results do not depend on the choice. */
 int lbm_ensemble_set_tuning(lbm_ensemble *ens, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs,
-                            int32_t flags);
+                            int32_t flags, int32_t cells_per_thread);
 int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs,
-                            int32_t *flags);
-/* Times each (block_x, blocks_per_cu, flags) candidate for `steps` steps on the live
- * ensemble, times the four fastest again (twice, over 3*steps steps each) and
- * keeps the fastest of those; with cache_path, a result stored for the same
- * precision and cavity-size multiset is reused without timing. */
+                            int32_t *flags, int32_t *cells_per_thread);
+/* Times each (block_x, blocks_per_cu, flags) shape at one cell per thread for
+ * `steps` steps on the live ensemble, then 2 and 4 cells per thread for the
+ * four fastest shapes, re-times the four fastest candidates (twice, over
+ * 3*steps steps each) and keeps the fastest of those; with cache_path, a
+ * result stored for the same precision and cavity-size multiset is reused
+ * without timing. */
 int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_path, double *best_ms_per_step);
 
 /* ---- Serving: continuous batching of arriving solves ----------------------
@@ -292,7 +296,7 @@ typedef struct lbm_serve_options {
     uint64_t hbm_budget;  /* bytes for resident cavities */
     double time_limit_s;
     int32_t block_x, blocks_per_cu, flags; /* ensemble launch shape; block_x 0: default */
-    int32_t reserved;
+    int32_t cells_per_thread;              /* with block_x > 0: 1, 2 or 4 (0: 1) */
     uint64_t max_resident_cells; /* 0: no cap; else admit while resident cells stay below it
                                     (enough cells to saturate HBM, so admitted requests finish) */
 } lbm_serve_options;

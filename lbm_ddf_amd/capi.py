@@ -114,9 +114,11 @@ SIGNATURES = {
     "lbm_ensemble_step": (ctypes.c_int, [_P, ctypes.c_int64]),
     "lbm_ensemble_step_timed": (ctypes.c_int, [_P, ctypes.c_int64, _D]),
     "lbm_ensemble_run_diag": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(Diag), ctypes.c_int32]),
-    "lbm_ensemble_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "lbm_ensemble_set_tuning": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32]),
     "lbm_ensemble_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
-                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.POINTER(ctypes.c_int32)]),
     "lbm_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, _D]),
 }
 
@@ -426,17 +428,17 @@ class Ensemble:
         _check(self.lib.lbm_ensemble_run_diag(self.h, int(n), arr, max(1, slots)))
         return {s: arr[s].as_dict() for s in self.params}
 
-    def set_tuning(self, block_x: int, blocks_per_cu: int, graph_pairs: int, flags: int = 7):
+    def set_tuning(self, block_x: int, blocks_per_cu: int, graph_pairs: int, flags: int = 7,
+                   cells_per_thread: int = 1):
         _check(self.lib.lbm_ensemble_set_tuning(self.h, int(block_x), int(blocks_per_cu), int(graph_pairs),
-                                                int(flags)))
+                                                int(flags), int(cells_per_thread)))
 
     @property
     def tuning(self) -> tuple:
-        """(block_x, blocks_per_cu, graph_pairs, flags)."""
-        a, b, c, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(self.lib.lbm_ensemble_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
-                                                ctypes.byref(d)))
-        return a.value, b.value, c.value, d.value
+        """(block_x, blocks_per_cu, graph_pairs, flags, cells_per_thread)."""
+        v = [ctypes.c_int32() for _ in range(5)]
+        _check(self.lib.lbm_ensemble_get_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, steps: int = 8, cache_path: str | None = None) -> float:
         ms = ctypes.c_double()
@@ -469,7 +471,7 @@ class Result(ctypes.Structure):
 class ServeOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("chunk_steps", ctypes.c_int64),
                 ("hbm_budget", ctypes.c_uint64), ("time_limit_s", ctypes.c_double), ("block_x", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("flags", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
                 ("max_resident_cells", ctypes.c_uint64)]
 
 

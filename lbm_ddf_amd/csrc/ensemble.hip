@@ -5,7 +5,8 @@
 // borrows the ensemble's stream, so every reference-state operation (finalize,
 // host copies, reductions) is the single-cavity code.  The ensemble adds:
 //
-//   * a flat tile space: tile = BX consecutive cells of one cavity; cavity k
+//   * a flat tile space: tile = cells_per_thread * BX consecutive cells of
+//     one cavity (each thread updates cells_per_thread cells); cavity k
 //     owns tiles [tile0_k, tile0_k + ntiles_k).  A device descriptor table
 //     (EnsCavity) holds each cavity's step arguments for both buffer
 //     parities, and a tile map gives each tile's cavity;
@@ -88,6 +89,7 @@ struct lbm_ensemble {
     int blocks_per_cu = 8;
     int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
     int flags = 7;        // launch variant of the plain step (0, 3, 4, 7)
+    int cpt = 1;          // cells per thread (1, 2, 4)
     int num_cus = 256;
     // graph of graph_pairs (parity 0, parity 1) launches
     hipGraphExec_t graph = nullptr;
@@ -165,7 +167,8 @@ int upload_tables(lbm_ensemble *e) {
         d.d.diff = c->d.diff;
         d.d.dT = c->p.t_hot - c->p.t_cold;
         d.d.step = 0;
-        d.ntiles = static_cast<uint32_t>((c->n + e->bx - 1) / e->bx);
+        const uint64_t tile_cells = static_cast<uint64_t>(e->bx) * e->cpt;
+        d.ntiles = static_cast<uint32_t>((c->n + tile_cells - 1) / tile_cells);
         d.tile0 = tile;
         d.active = c->phase == lbm::PHASE_FUSED ? 1 : 0;
         d.slot = static_cast<int32_t>(s);
@@ -191,7 +194,8 @@ int upload_tables(lbm_ensemble *e) {
     e->ncav = ncav;
     e->ctrl_host[0] = ncav;
     e->ctrl_host[1] = static_cast<uint32_t>(tiles);
-    e->ctrl_host[2] = e->ctrl_host[3] = 0;
+    e->ctrl_host[2] = static_cast<uint32_t>(e->cpt);
+    e->ctrl_host[3] = 0;
     if (ncav) ENS_HIP(hipMemcpyAsync(e->cav_dev, e->cav_host, sizeof(lbm::EnsCavity) * ncav, hipMemcpyHostToDevice, e->stream));
     ENS_HIP(hipMemcpyAsync(e->ctrl_dev, e->ctrl_host, 4 * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
     e->tables_dirty = false;
@@ -441,29 +445,35 @@ int lbm_ensemble_cavity(lbm_ensemble *e, int32_t slot, lbm_ctx **ctx) {
 }
 
 int lbm_ensemble_set_tuning(lbm_ensemble *e, int32_t block_x, int32_t blocks_per_cu, int32_t graph_pairs,
-                            int32_t flags) {
+                            int32_t flags, int32_t cells_per_thread) {
     if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
     if (block_x != 64 && block_x != 128 && block_x != 256) return ctx_fail(LBM_E_INVALID, "block_x must be 64, 128 or 256");
     if (blocks_per_cu < 1 || blocks_per_cu > 64 || graph_pairs < 0 || graph_pairs > 256)
         return ctx_fail(LBM_E_INVALID, "blocks_per_cu in 1..64, graph_pairs in 0..256");
     if (flags != 0 && flags != 3 && flags != 4 && flags != 7) return ctx_fail(LBM_E_INVALID, "flags must be 0, 3, 4 or 7");
-    if (block_x != e->bx) e->tables_dirty = true;  // tile count per cavity changes
-    if (block_x != e->bx || blocks_per_cu != e->blocks_per_cu || graph_pairs != e->graph_pairs || flags != e->flags)
+    if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
+        return ctx_fail(LBM_E_INVALID, "cells_per_thread must be 1, 2 or 4");
+    // the tile count per cavity changes with the tile size
+    if (block_x != e->bx || cells_per_thread != e->cpt) e->tables_dirty = true;
+    if (block_x != e->bx || blocks_per_cu != e->blocks_per_cu || graph_pairs != e->graph_pairs || flags != e->flags ||
+        cells_per_thread != e->cpt)
         drop_graph(e);
     e->bx = block_x;
     e->blocks_per_cu = blocks_per_cu;
     e->graph_pairs = graph_pairs;
     e->flags = flags;
+    e->cpt = cells_per_thread;
     return LBM_OK;
 }
 
 int lbm_ensemble_get_tuning(const lbm_ensemble *e, int32_t *block_x, int32_t *blocks_per_cu, int32_t *graph_pairs,
-                            int32_t *flags) {
+                            int32_t *flags, int32_t *cells_per_thread) {
     if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
     if (block_x) *block_x = e->bx;
     if (blocks_per_cu) *blocks_per_cu = e->blocks_per_cu;
     if (graph_pairs) *graph_pairs = e->graph_pairs;
     if (flags) *flags = e->flags;
+    if (cells_per_thread) *cells_per_thread = e->cpt;
     return LBM_OK;
 }
 
@@ -472,60 +482,73 @@ int lbm_ensemble_autotune(lbm_ensemble *e, int32_t steps, const char *cache_path
     const std::string key = tuning_key(e);
     const int gp = std::max(1, steps / 2);
     if (cache_path && *cache_path) {
+        // one line per tuned ensemble: key block_x blocks_per_cu flags cells_per_thread ms
         std::ifstream in(cache_path);
-        std::string k;
-        int bx, bpc, fl;
-        double ms;
-        while (in >> k >> bx >> bpc >> fl >> ms) {
-            if (k == key) {
-                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp, fl)) return rc;
-                if (best_ms_per_step) *best_ms_per_step = ms;
-                return LBM_OK;
-            }
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            int bx, bpc, fl, cpt;
+            double ms;
+            if (!(ls >> k >> bx >> bpc >> fl >> cpt >> ms) || k != key) continue;
+            if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp, fl, cpt)) return rc;
+            if (best_ms_per_step) *best_ms_per_step = ms;
+            return LBM_OK;
         }
     }
-    const int blocks[] = {64, 128, 256};
-    const int bpcs[] = {4, 8, 16};
-    const int flagset[] = {0, 3, 4, 7};
     struct Cand {
-        int bx, bpc, fl;
+        int bx, bpc, fl, cpt;
         double ms;
     };
+    auto time_short = [&](Cand &c) -> int {
+        if (int rc = lbm_ensemble_set_tuning(e, c.bx, c.bpc, gp, c.fl, c.cpt)) return rc;
+        if (int rc = lbm_ensemble_step(e, 2 * gp)) return rc;  // builds the graph, warms caches
+        return lbm_ensemble_step_timed(e, 2 * gp, &c.ms);
+    };
+    auto by_time = [](const Cand &a, const Cand &b) { return a.ms < b.ms; };
+    // Stage 1: launch shapes at one cell per thread.
     std::vector<Cand> cands;
-    for (int bx : blocks)
-        for (int bpc : bpcs)
-            for (int fl : flagset) {
-                if (int rc = lbm_ensemble_set_tuning(e, bx, bpc, gp, fl)) return rc;
-                if (int rc = lbm_ensemble_step(e, 2 * gp)) return rc;  // builds the graph, warms caches
-                double ms = 0.0;
-                if (int rc = lbm_ensemble_step_timed(e, 2 * gp, &ms)) return rc;
-                cands.push_back({bx, bpc, fl, ms});
+    for (int bx : {64, 128, 256})
+        for (int bpc : {4, 8, 16})
+            for (int fl : {0, 3, 4, 7}) {
+                Cand c{bx, bpc, fl, 1, 0.0};
+                if (int rc = time_short(c)) return rc;
+                cands.push_back(c);
             }
+    // Stage 2: more cells per thread for the four fastest shapes.
+    std::sort(cands.begin(), cands.end(), by_time);
+    const size_t shapes = std::min<size_t>(4, cands.size());
+    for (size_t i = 0; i < shapes; ++i)
+        for (int cpt : {2, 4}) {
+            Cand c = cands[i];
+            c.cpt = cpt;
+            if (int rc = time_short(c)) return rc;
+            cands.push_back(c);
+        }
     // Short timings scatter by several percent from run to run: the fastest
-    // few are timed again over three times as many steps, twice, and the
+    // four are timed again over three times as many steps, twice, and the
     // best of those (each candidate's better run) is kept.
-    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    std::sort(cands.begin(), cands.end(), by_time);
     const size_t finalists = std::min<size_t>(4, cands.size());
-    Cand best{e->bx, e->blocks_per_cu, e->flags, 1e300};
+    Cand best{e->bx, e->blocks_per_cu, e->flags, e->cpt, 1e300};
     for (size_t i = 0; i < finalists; ++i) {
-        Cand &c = cands[i];
-        if (int rc = lbm_ensemble_set_tuning(e, c.bx, c.bpc, gp, c.fl)) return rc;
+        const Cand &c = cands[i];
+        if (int rc = lbm_ensemble_set_tuning(e, c.bx, c.bpc, gp, c.fl, c.cpt)) return rc;
         double run_best = 1e300;
         for (int rep = 0; rep < 2; ++rep) {
             double ms = 0.0;
             if (int rc = lbm_ensemble_step_timed(e, 6 * gp, &ms)) return rc;
             run_best = std::min(run_best, ms / 3.0);
         }
-        if (run_best < best.ms) best = Cand{c.bx, c.bpc, c.fl, run_best};
+        if (run_best < best.ms) best = Cand{c.bx, c.bpc, c.fl, c.cpt, run_best};
     }
-    const int best_bx = best.bx, best_bpc = best.bpc, best_fl = best.fl;
-    if (int rc = lbm_ensemble_set_tuning(e, best_bx, best_bpc, gp, best_fl)) return rc;
-    double best_ms = best.ms / (2 * gp);
+    if (int rc = lbm_ensemble_set_tuning(e, best.bx, best.bpc, gp, best.fl, best.cpt)) return rc;
+    const double best_ms = best.ms / (2 * gp);
     if (best_ms_per_step) *best_ms_per_step = best_ms;
     if (cache_path && *cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o.precision(17);
-        o << key << " " << best_bx << " " << best_bpc << " " << best_fl << " " << best_ms << "\n";
+        o << key << " " << best.bx << " " << best.bpc << " " << best.fl << " " << best.cpt << " " << best_ms << "\n";
     }
     return LBM_OK;
 }

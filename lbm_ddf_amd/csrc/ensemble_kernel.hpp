@@ -20,6 +20,7 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
                                                       const uint32_t *__restrict__ ctrl,
                                                       const uint32_t *__restrict__ tmap, int parity) {
     const uint32_t ntiles = ctrl[1];
+    const uint32_t cpt = ctrl[2];  // cells per thread: a tile is cpt * BX consecutive cells
     // Tile order.  Plain: block b takes tiles b, b+G, ...  XCD-contiguous:
     // the blocks of XCD x (b % 8 == x) share the x-th eighth of the tile
     // space, so neighbouring tiles -- which share the cache lines straddled by
@@ -48,19 +49,27 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
         if (!c.active) continue;
         const StepArgs &a = c.a[parity];
         const uint32_t lt = t - c.tile0;
-        const uint32_t cell = lt * BX + threadIdx.x;
         double q[NQ];
         if constexpr (REDUCE) {
 #pragma unroll
             for (int k = 0; k < NQ; ++k) q[k] = 0.0;
         }
-        if (cell < a.n) {
+        for (uint32_t k = 0; k < cpt; ++k) {
+            const uint32_t cell = (lt * cpt + k) * BX + threadIdx.x;
+            if (cell >= a.n) break;
             const uint32_t lx = static_cast<uint32_t>(a.lx), ly = static_cast<uint32_t>(a.ly);
             const uint32_t r = cell / lx;
             const int ix = static_cast<int>(cell - r * lx);
             const int iy = static_cast<int>(r % ly);
             const int iz = static_cast<int>(r / ly);
-            step_cell<P, MODE, REDUCE, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, q);
+            if constexpr (REDUCE) {
+                double qc[NQ];
+                step_cell<P, MODE, true, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, qc);
+#pragma unroll
+                for (int j = 0; j < NQ; ++j) q[j] = (j == Q_MAXUSQ) ? fmax(q[j], qc[j]) : q[j] + qc[j];
+            } else {
+                step_cell<P, MODE, false, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, q);
+            }
         }
         if constexpr (REDUCE) block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(lt) * NQ);
     }

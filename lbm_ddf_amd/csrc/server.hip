@@ -71,7 +71,8 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
     lbm_ensemble *ens = nullptr;
     if (int rc = lbm_ensemble_create(opt->precision, &ens)) return rc;
     if (opt->block_x > 0)
-        if (int rc = lbm_ensemble_set_tuning(ens, opt->block_x, opt->blocks_per_cu, 8, opt->flags)) {
+        if (int rc = lbm_ensemble_set_tuning(ens, opt->block_x, opt->blocks_per_cu, 8, opt->flags,
+                                             opt->cells_per_thread > 0 ? opt->cells_per_thread : 1)) {
             lbm_ensemble_destroy(ens);
             return rc;
         }

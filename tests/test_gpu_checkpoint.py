@@ -69,8 +69,9 @@ def run_rb3d(outdir, *extra):
     return out.stdout
 
 
-def test_rb3d_driver_outputs_match_the_oracle(tmp_path):
-    log = run_rb3d(tmp_path, "--nend", "40")
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_rb3d_driver_outputs_match_the_oracle(tmp_path, pattern):
+    log = run_rb3d(tmp_path, "--nend", "40", "--pattern", pattern)
     assert "Simulation completed." in log and "Step 20" in log
     d = tmp_path / "Ra1.0e+05Pr0.71"
     o = oracle.OracleState(12, 10, 8, **RB)
@@ -94,11 +95,12 @@ def test_rb3d_driver_outputs_match_the_oracle(tmp_path):
 
 
 def test_rb3d_continue_equals_straight_run(tmp_path):
+    """A checkpoint written by one storage pattern restarts in the other."""
     a, b = tmp_path / "a", tmp_path / "b"
     a.mkdir()
     b.mkdir()
     run_rb3d(a, "--nend", "60")
-    run_rb3d(b, "--nend", "40")
+    run_rb3d(b, "--nend", "40", "--pattern", "aa")
     run_rb3d(b, "--nend", "60", "--continue", "40")
     fa = (a / "Ra1.0e+05Pr0.71" / "fg000000060.dat").read_bytes()
     fb = (b / "Ra1.0e+05Pr0.71" / "fg000000060.dat").read_bytes()

@@ -48,10 +48,10 @@ def assert_member_equals(e, slot, cav, precision="fp64"):
     return k
 
 
-@pytest.mark.parametrize("graph_pairs", [0, 4])
-def test_members_equal_standalone_runs(graph_pairs):
+@pytest.mark.parametrize("graph_pairs,cpt", [(0, 1), (4, 1), (4, 2), (0, 4)])
+def test_members_equal_standalone_runs(graph_pairs, cpt):
     e, slots = build()
-    e.set_tuning(128, 4, graph_pairs, 7 if graph_pairs else 0)
+    e.set_tuning(128, 4, graph_pairs, 7 if graph_pairs else 0, cpt)
     e.step(37)
     for s, cav in zip(slots, CAVS):
         # prologue (collides step 1) + 37 launches
@@ -73,8 +73,10 @@ def test_graph_replays_with_eager_work_between():
     e.close()
 
 
-def test_run_diag_matches_standalone_reduce():
+@pytest.mark.parametrize("cpt", [1, 4])
+def test_run_diag_matches_standalone_reduce(cpt):
     e, slots = build()
+    e.set_tuning(64, 8, 4, 7, cpt)
     e.step(10)
     diags = e.run_diag(5)
     for s, cav in zip(slots, CAVS):
