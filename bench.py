@@ -197,7 +197,7 @@ def run_serve_workload(args) -> dict:
         f"cells, precision {args.serve_precision}, time limit {args.serve_time_limit:.0f} s")
     res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
                     hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit,
-                    max_resident_cells=int(args.serve_max_cells))
+                    max_resident_cells=int(args.serve_max_cells), progress_s=30.0)
     log(f"serve: {st['completed']} completed, {st['unfinished']} unfinished, {st['pending']} pending, "
         f"{st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
         f"p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; peak resident {st['peak_resident']}")
@@ -336,6 +336,8 @@ def main() -> int:
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage leg of the default workload")
+    ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
     ap.add_argument("--workload", default="single", choices=["single", "ensemble", "serve", "precision", "precision-report"],
@@ -506,8 +508,47 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.ra, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    if prec == "fp64" and not args.no_fp16:
+        out["fp16_storage"] = fp16_leg(args, mlups)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def fp16_leg(args, fp64_mlups: float) -> dict:
+    """The same cavity with FP16 storage (FP32 arithmetic): MLUPS, roofline
+    fraction at 216 B per cell and the speed-up over the FP64 run above."""
+    from lbm_ddf_amd import Cavity
+
+    L = args.size
+    n = L ** 3
+    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision="fp16",
+               pattern=args.pattern)
+    c.init_conductive(2, 1.0e-3)
+    tuned = c.autotune(max(1, args.autotune)) if args.autotune > 0 else None
+    c.step(300)
+    c.synchronize()
+    steps = max(1, args.fp16_steps)
+    t0 = time.perf_counter()
+    kernel_ms = c.step_timed(steps)
+    c.synchronize()
+    wall = time.perf_counter() - t0
+    d = c.reduce()
+    bx, fl = c.tuning
+    c.close()
+    mlups = n * steps / wall / 1e6
+    achieved = BYTES_PER_CELL["fp16"] * n / (kernel_ms / 1e3 / steps) / 1e9
+    log(f"fp16 storage: {mlups:.0f} MLUPS ({mlups / fp64_mlups:.2f}x fp64), {achieved:.0f} GB/s")
+    return {"value": round(mlups, 2), "unit": "MLUPS", "steps": steps, "ms_per_step": round(wall / steps * 1e3, 4),
+            "speedup_vs_fp64": round(mlups / fp64_mlups, 3),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_from_profiles(f"fused_fp16_{L}_bx{bx}_f{fl}")
+                         or traffic_from_profiles(f"fused_fp16_{L}"),
+                         "algorithmic_bytes_per_launch": BYTES_PER_CELL["fp16"] * n},
+            "launch": {"block_x": bx, "flags": fl, "autotuned": tuned is not None},
+            "nonfinite": d["nonfinite"],
+            "note": "same workload with populations stored as FP16 deviations, FP32 arithmetic; "
+                    "accuracy against FP64 in the precision study (bench.py --workload precision)"}
 
 
 if __name__ == "__main__":

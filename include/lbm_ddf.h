@@ -299,6 +299,7 @@ typedef struct lbm_serve_options {
     int32_t cells_per_thread;              /* with block_x > 0: 1, 2 or 4 (0: 1) */
     uint64_t max_resident_cells; /* 0: no cap; else admit while resident cells stay below it
                                     (enough cells to saturate HBM, so admitted requests finish) */
+    double progress_s;           /* > 0: the loop prints a progress line to stderr at this interval */
 } lbm_serve_options;
 
 typedef struct lbm_serve_stats {

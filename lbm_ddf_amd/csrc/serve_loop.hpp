@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "lbm_ddf.h"
@@ -93,8 +94,8 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
     std::vector<lbm_diag> diag;
     std::vector<int64_t> before;
     uint64_t bytes_used = 0, peak_bytes = 0, cells_used = 0;
-    int32_t peak_resident = 0;
-    double cell_updates = 0.0, busy = 0.0;
+    int32_t peak_resident = 0, retired = 0;
+    double cell_updates = 0.0, busy = 0.0, next_report = opt->progress_s;
     int rc = LBM_OK;
 
     auto finish = [&](Live &l, int32_t status, const lbm_diag *d, int64_t steps) {
@@ -109,6 +110,7 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
             r.kinetic_energy = d->kinetic_energy;
         }
         r.nu_running = l.run_nu;
+        ++retired;
         be.remove(l.slot);
         bytes_used -= l.bytes;
         cells_used -= l.cells;
@@ -202,6 +204,14 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
             if (done >= q.max_steps) finish(l, LBM_REQ_BUDGET, &d, done);
         }
         live.erase(std::remove_if(live.begin(), live.end(), [](const Live &l) { return l.req < 0; }), live.end());
+        if (opt->progress_s > 0 && clk.now() >= next_report) {
+            const double t = clk.now();
+            std::fprintf(stderr, "[serve] %.0f s: %zu resident, %zu queued, %zu not yet arrived, %d retired, %.0f MLUPS\n",
+                         t, live.size(), waiting.size(), order.size() - next_arrival, retired,
+                         t > 0 ? cell_updates / t / 1e6 : 0.0);
+            std::fflush(stderr);
+            next_report = t + opt->progress_s;
+        }
     }
 
     // ---- end of run: whatever is resident did not finish -------------------
