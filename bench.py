@@ -329,6 +329,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--lz", type=int, default=0,
+                    help="single workload: z extent (default: --size), e.g. 64 for one GPU's "
+                         "512x512x64 share of a z-slab split over 8 GPUs")
     ap.add_argument("--ra", type=float, default=1.0e7)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp16", "bf16"])
     ap.add_argument("--pattern", default="ab", choices=["ab", "aa"],
@@ -423,11 +426,13 @@ def main() -> int:
     from lbm_ddf_amd import Cavity, bytes_required, make_params
 
     L = args.size
-    n = L ** 3
+    lz = args.lz if args.lz > 0 else L
+    n = L * L * lz
+    shape = f"{L}" if lz == L else f"{L}x{L}x{lz}"
     prec = args.precision
-    need = bytes_required(make_params(L, L, L, precision=prec, pattern=args.pattern))
-    log(f"cavity {L}^3 {prec} {args.pattern}: {need / 1e9:.1f} GB device memory")
-    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec,
+    need = bytes_required(make_params(L, L, lz, precision=prec, pattern=args.pattern))
+    log(f"cavity {L}x{L}x{lz} {prec} {args.pattern}: {need / 1e9:.1f} GB device memory")
+    c = Cavity(L, L, lz, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec,
                pattern=args.pattern)
     c.init_conductive(2, 1.0e-3)
     c.synchronize()
@@ -460,7 +465,7 @@ def main() -> int:
     mlups = n * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL[prec] * n / avg_launch_s / 1e9
-    key = f"fused_{prec}_{L}" + ("_aa" if args.pattern == "aa" else "")
+    key = f"fused_{prec}_{shape}" + ("_aa" if args.pattern == "aa" else "")
     # traffic was measured per launch variant; fall back to the workload's figure
     traffic = traffic_from_profiles(f"{key}_bx{c_tuning[0]}_f{c_tuning[1]}")
     if traffic is None:
@@ -480,7 +485,7 @@ def main() -> int:
         "dtype": "f64" if prec == "fp64" else f"{prec} storage, f32 arithmetic",
         "data": "synthetic: conductive start with seeded +-1e-3 perturbation",
         "config": {
-            "workload": f"Rayleigh-Benard cavity {L}x{L}x{L}, Ra={args.ra:g}, Pr=0.71, walls +-0.5, "
+            "workload": f"Rayleigh-Benard cavity {L}x{L}x{lz}, Ra={args.ra:g}, Pr=0.71, walls +-0.5, "
                         f"{prec} populations, fused collide+stream of f and g"
                         + (", one buffer updated in place (AA pattern)" if args.pattern == "aa" else ""),
             "cells": n,
@@ -520,8 +525,10 @@ def fp16_leg(args, fp64_mlups: float) -> dict:
     from lbm_ddf_amd import Cavity
 
     L = args.size
-    n = L ** 3
-    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision="fp16",
+    lz = args.lz if args.lz > 0 else L
+    n = L * L * lz
+    shape = (f"{L}" if lz == L else f"{L}x{L}x{lz}") + ("_aa" if args.pattern == "aa" else "")
+    c = Cavity(L, L, lz, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision="fp16",
                pattern=args.pattern)
     c.init_conductive(2, 1.0e-3)
     tuned = c.autotune(max(1, args.autotune)) if args.autotune > 0 else None
@@ -542,8 +549,8 @@ def fp16_leg(args, fp64_mlups: float) -> dict:
             "speedup_vs_fp64": round(mlups / fp64_mlups, 3),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": traffic_from_profiles(f"fused_fp16_{L}_bx{bx}_f{fl}")
-                         or traffic_from_profiles(f"fused_fp16_{L}"),
+                         "traffic": traffic_from_profiles(f"fused_fp16_{shape}_bx{bx}_f{fl}")
+                         or traffic_from_profiles(f"fused_fp16_{shape}"),
                          "algorithmic_bytes_per_launch": BYTES_PER_CELL["fp16"] * n},
             "launch": {"block_x": bx, "flags": fl, "autotuned": tuned is not None},
             "nonfinite": d["nonfinite"],

@@ -46,7 +46,8 @@ extern "C" {
 
 /* Storage precision of the populations. */
 #define LBM_PREC_FP64 0 /* FP64 storage, FP64 arithmetic */
-#define LBM_PREC_FP16 1 /* FP16 deviations (scaled), FP32 arithmetic */
+#define LBM_PREC_FP16 1 /* FP16 deviations (scaled), FP32 arithmetic;
+                           needs |t_hot|, |t_cold| <= 1 */
 #define LBM_PREC_BF16 2 /* BF16 deviations, FP32 arithmetic */
 
 /* Field selectors for lbm_get_field / lbm_set_field

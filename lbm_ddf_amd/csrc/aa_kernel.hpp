@@ -111,8 +111,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             if constexpr (CZ[i] < 0) jz = jz - f[i];
             phi = phi + g[i];
         });
-        const T phi_prev = static_cast<T>(lag[idx]);
-        const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
+        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
         ux = jx;
         uy = jy + T(0.5) * fprev;
         uz = jz;
@@ -128,7 +127,8 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
 
     if constexpr (MODE != MODE_AA_MOMENTS) {
         // forcing (src/macrovar.cu:45) and collision
-        const T F = static_cast<T>(a.grav_beta) * (phi - t0);
+        const T dphi = phi - t0;
+        const T F = static_cast<T>(a.grav_beta) * dphi;
         collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
         const T rw = rho;  // rho' of this step: read by the isothermal walls of the next streaming
         const T tw = static_cast<T>(iy == 0 ? a.t_hot : a.t_cold);
@@ -164,7 +164,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                 st<NTS>(&gs[dst], P::template enc_g<j>(gv, tr));
             }
         });
-        lag[idx] = static_cast<L>(phi);
+        lag[idx] = P::enc_side(dphi);
     }
 
     if constexpr (REDUCE && MODE != MODE_AA_PROLOGUE) {

@@ -46,7 +46,8 @@ int hip_fail(hipError_t e, const char *what) {
 
 
 size_t elem_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 2; }
-size_t side_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : 4; }
+// lag phi and wall rho': FP64, scaled FP16 (FP16 storage) or FP32 (BF16 storage)
+size_t side_size(int prec) { return prec == LBM_PREC_FP64 ? 8 : (prec == LBM_PREC_FP16 ? 2 : 4); }
 
 int default_block(int lx) {
     if (lx <= 64) return 64;
@@ -63,7 +64,11 @@ int validate(const lbm_params *p) {
         return fail(LBM_E_INVALID, "unknown precision");
     if (p->pattern != LBM_PATTERN_AB && p->pattern != LBM_PATTERN_AA)
         return fail(LBM_E_INVALID, "unknown pattern");
+    if (p->reserved != 0) return fail(LBM_E_INVALID, "reserved must be 0");
     if (!(p->rayleigh > 0.0) || !(p->prandtl > 0.0)) return fail(LBM_E_INVALID, "Ra and Pr must be > 0");
+    // the scaled binary16 code holds magnitudes below 2 (step_kernel.hpp, PolicyF16)
+    if (p->precision == LBM_PREC_FP16 && !(std::fabs(p->t_hot) <= 1.0 && std::fabs(p->t_cold) <= 1.0))
+        return fail(LBM_E_INVALID, "FP16 storage needs |t_hot|, |t_cold| <= 1");
     return LBM_OK;
 }
 

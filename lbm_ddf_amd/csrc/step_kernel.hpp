@@ -14,9 +14,9 @@
 // plus, optionally, the per-cell terms of the diagnostics (src/output.cu).
 //
 // Two per-cell side values carry what the reference keeps in full fields:
-//   lag[x]      phi of the previous completed step, because the reference
-//               adds half of the force computed from the previous phi to u
-//               (src/macrovar.cu:31 with src/main.cu:27);
+//   lag[x]      phi - t0 of the previous completed step, because the
+//               reference adds half of the force computed from the previous
+//               phi to u (src/macrovar.cu:31 with src/main.cu:27);
 //   wall_rho[x] rho' of the previous completed step on the planes iy = 0 and
 //               iy = LY-1, read by the isothermal anti-bounce-back
 //               (src/streaming.cu:206-227).
@@ -46,6 +46,8 @@ struct PolicyF64 {
     static constexpr bool exact = true;
     __device__ __forceinline__ static T dec_f(S s) { return s; }
     __device__ __forceinline__ static S enc_f(T t) { return t; }
+    __device__ __forceinline__ static T dec_side(L s) { return s; }
+    __device__ __forceinline__ static L enc_side(T t) { return t; }
     template <int I>
     __device__ __forceinline__ static T dec_g(S s, T) { return s; }
     template <int I>
@@ -57,15 +59,22 @@ struct PolicyF64 {
 // dec_g/enc_g take the row's reference temperature: g is stored as
 // g_i - w_i*T_ref so that near the isothermal walls, where the Nusselt
 // gradients live, the stored values are small deviations, not |T| ~ 0.5.
+// The side values (lag phi - t0, bounded by |T_hot - T_cold| / 2, and wall
+// rho') use the same scaled binary16.  FP16 storage is accepted only for
+// |T_hot|, |T_cold| <= 1: then the stored deviations and side values stay
+// below 2 in magnitude while |u| is below the sound speed, and their scaled
+// values below the binary16 maximum (65504).
 struct PolicyF16 {
     using S = _Float16;
     using T = float;
-    using L = float;
+    using L = _Float16;
     static constexpr bool exact = false;
     static constexpr float kScale = 32768.0f;
     static constexpr float kInv = 1.0f / 32768.0f;
     __device__ __forceinline__ static T dec_f(S s) { return static_cast<float>(s) * kInv; }
     __device__ __forceinline__ static S enc_f(T t) { return static_cast<_Float16>(t * kScale); }
+    __device__ __forceinline__ static T dec_side(L s) { return static_cast<float>(s) * kInv; }
+    __device__ __forceinline__ static L enc_side(T t) { return static_cast<_Float16>(t * kScale); }
     template <int I>
     __device__ __forceinline__ static T dec_g(S s, T tref) {
         return static_cast<float>(s) * kInv + static_cast<float>(weight(I)) * tref;
@@ -94,6 +103,8 @@ struct PolicyBF16 {
     static constexpr bool exact = false;
     __device__ __forceinline__ static T dec_f(S s) { return bf16_bits_to_float(s); }
     __device__ __forceinline__ static S enc_f(T t) { return float_to_bf16_bits(t); }
+    __device__ __forceinline__ static T dec_side(L s) { return s; }
+    __device__ __forceinline__ static L enc_side(T t) { return t; }
     template <int I>
     __device__ __forceinline__ static T dec_g(S s, T tref) {
         return bf16_bits_to_float(s) + static_cast<float>(weight(I)) * tref;
@@ -427,7 +438,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             // Isothermal anti-bounce-back at the y walls for g.  The reference
             // applies the z walls after the y walls, so a direction whose
             // source is also outside in z keeps the adiabatic bounce-back.
-            const T rw = static_cast<T>(rw_raw);
+            const T rw = P::dec_side(rw_raw);
             const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
@@ -457,8 +468,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             if constexpr (CZ[i] < 0) jz = jz - f[i];
             phi = phi + g[i];
         });
-        const T phi_prev = static_cast<T>(lag[idx]);
-        const T fprev = static_cast<T>(a.grav_beta) * (phi_prev - t0);
+        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
         ux = jx;
         uy = jy + T(0.5) * fprev;
         uz = jz;
@@ -479,15 +489,16 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
 
     if constexpr (MODE != MODE_FINALIZE) {
         // ---- forcing (src/macrovar.cu:45) and collision -----------------
-        const T F = static_cast<T>(a.grav_beta) * (phi - t0);
+        const T dphi = phi - t0;
+        const T F = static_cast<T>(a.grav_beta) * dphi;
         collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
             st<NTS>(&gb[at + i * ps], P::template enc_g<i>(g[i], tref_y));
         });
-        lag[idx] = static_cast<L>(phi);
-        if (wall_lo || wall_hi) wall_rho[wall_idx] = static_cast<L>(rho);
+        lag[idx] = P::enc_side(dphi);
+        if (wall_lo || wall_hi) wall_rho[wall_idx] = P::enc_side(rho);
     }
 
     if constexpr (REDUCE && MODE != MODE_PROLOGUE) {

@@ -90,6 +90,30 @@ def test_create_rejects_bad_precision_and_physics():
     p.pattern = 2
     assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
     assert b"pattern" in lib.lbm_last_error()
+    p = capi.make_params(8, 8, 8)
+    p.reserved = 1
+    assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"reserved" in lib.lbm_last_error()
+    assert not h.value
+
+
+@pytest.mark.parametrize("walls", [(1.5, 0.0), (0.0, -2.0), (float("nan"), 0.0)])
+def test_fp16_storage_rejects_wall_temperatures_beyond_its_range(walls):
+    """Scaled binary16 holds magnitudes below 2: FP16 storage takes wall
+    temperatures in [-1, 1] (the reference default is 1 and 0); FP64 and BF16
+    take any."""
+    lib = capi.load()
+    h = ctypes.c_void_p()
+    p = capi.make_params(8, 8, 8, precision="fp16", t_hot=walls[0], t_cold=walls[1])
+    assert lib.lbm_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"FP16" in lib.lbm_last_error()
+    assert not h.value
+    nbytes = ctypes.c_uint64()
+    for prec in ("fp64", "bf16"):
+        q = capi.make_params(8, 8, 8, precision=prec, t_hot=walls[0], t_cold=walls[1])
+        assert lib.lbm_bytes_required(ctypes.byref(q), ctypes.byref(nbytes)) == 0
+    q = capi.make_params(8, 8, 8, precision="fp16", t_hot=1.0, t_cold=-1.0)
+    assert lib.lbm_bytes_required(ctypes.byref(q), ctypes.byref(nbytes)) == 0
 
 
 def test_null_context_is_an_error_not_a_crash():
