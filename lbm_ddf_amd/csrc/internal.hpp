@@ -52,7 +52,7 @@ struct StepArgs {
     // 16-bit storage keeps g_i - w_i*T_ref(iy), with T_ref the conductive
     // profile between the walls: T_ref(iy) = tref0 - iy*dtref.
     double tref0, dtref;
-    void *lag;            // phi of the previous completed step [n]
+    void *lag;            // phi - t0 of the previous completed step [n]
     void *wall_rho;       // rho' of the previous completed step at iy=0 and iy=LY-1 [2][lz][lx]
     // reference-state fields (prologue reads, finalize writes)
     double *rho, *ux, *uy, *uz, *phi, *fy;
