@@ -78,7 +78,7 @@ def test_each_stage_bitwise(shape):
 
 
 @pytest.mark.parametrize("nsteps", [1, 2, 5, 23])
-@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (130, 5, 3), (256, 5, 3)])
+@pytest.mark.parametrize("shape", [(6, 5, 7), (65, 9, 4), (1, 4, 3), (3, 3, 1), (2, 5, 2), (130, 5, 3), (256, 5, 3)])
 def test_fused_step_bitwise(shape, nsteps):
     c, o = make_pair(*shape, seed=3, amp=5.0e-2, rayleigh=1.0e6)
     c.step(nsteps)

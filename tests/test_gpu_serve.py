@@ -88,3 +88,13 @@ def test_reject_divergence_and_time_limit():
     assert res[0]["status"] == "unfinished" and res[0]["steps"] > 0
     assert res[1]["status"] == "pending"
     assert st["unfinished"] == 1 and st["pending"] == 1
+
+
+def test_fp16_request_outside_the_codec_range_is_rejected_not_run():
+    """FP16 storage holds wall temperatures in [-1, 1]: a request outside it
+    is rejected at admission (the create-time check), the others run."""
+    hot = dict(A, t_hot=1.5, t_cold=0.0)
+    res, st = serve([hot, D], precision="fp16", chunk_steps=50)
+    assert res[0]["status"] == "rejected"
+    assert res[1]["status"] == "budget" and res[1]["steps"] == 300
+    assert st["rejected"] == 1 and st["completed"] == 1
