@@ -17,6 +17,7 @@
 // (aa_kernel.hpp): REF phase always holds layout N; in FUSED phase aa_layout
 // says whether the next pass is odd (layout E) or even (layout N).  Finalize
 // is the moments pass, preceded by the in-place exchange when the layout is E.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -303,7 +304,11 @@ bool valid_flags(bool aa, int flags) {
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     const bool aa = p->pattern == LBM_PATTERN_AA;
-    const uint64_t pops = (aa ? 1ull : 2ull) * lbm::pop_layout(p->lx, p->ly, p->lz).elems * elem_size(p->precision);
+    // the larger of the two layouts: a single cavity's or an ensemble member's
+    const size_t es = elem_size(p->precision);
+    const uint64_t elems = std::max(lbm::pop_layout(p->lx, p->ly, p->lz, es, false).elems,
+                                    lbm::pop_layout(p->lx, p->ly, p->lz, es, true).elems);
+    const uint64_t pops = (aa ? 1ull : 2ull) * elems * es;
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + (aa ? 0ull : 2ull * p->lx * p->lz * side_size(p->precision));
     (void)bx;
@@ -347,7 +352,7 @@ int lbm_destroy(lbm_ctx *c) {
     return LBM_OK;
 }
 
-int lbm_create(const lbm_params *p, lbm_ctx **out) { return lbm::ctx_create(p, nullptr, out); }
+int lbm_create(const lbm_params *p, lbm_ctx **out) { return lbm::ctx_create(p, nullptr, false, out); }
 
 }  // extern "C"
 
@@ -378,8 +383,9 @@ int ctx_forcing(lbm_ctx *c) {
 }
 
 // Creates a context; with stream != nullptr the context borrows that stream
-// (an ensemble's) instead of creating its own.
-int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
+// (an ensemble's) instead of creating its own.  flat_tiles: the context is an
+// ensemble member, stepped in flat tiles (selects its population layout).
+int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx **out) {
     int rc = validate(p);
     if (rc) return rc;
     if (!out) return fail(LBM_E_INVALID, "out is NULL");
@@ -389,7 +395,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out) {
     c->prec = p->precision;
     c->aa = p->pattern == LBM_PATTERN_AA;
     c->n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
-    const lbm::PopLayout lay = lbm::pop_layout(p->lx, p->ly, p->lz);
+    const lbm::PopLayout lay = lbm::pop_layout(p->lx, p->ly, p->lz, elem_size(p->precision), flat_tiles);
     c->ps = lay.ps;
     c->rs = lay.rs;
     derive(p, &c->d, &c->beta);

@@ -355,7 +355,7 @@ int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double
     if (p->pattern != LBM_PATTERN_AB)
         return ctx_fail(LBM_E_UNSUPPORTED, "ensemble members use the A/B pattern");
     lbm_ctx *c = nullptr;
-    if (int rc = lbm::ctx_create(p, e->stream, &c)) return rc;
+    if (int rc = lbm::ctx_create(p, e->stream, true, &c)) return rc;
     if (int rc = lbm_init_conductive(c, seed, amp)) {
         lbm_destroy(c);
         return rc;

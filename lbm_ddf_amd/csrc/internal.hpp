@@ -126,22 +126,25 @@ hipError_t launch_encode_plane(int precision, int which, int ip, const double *s
 //     concurrent streams lie in 54 distinct physical pages and their channel
 //     spread depended on where the driver placed each page: measured on
 //     MI355X, the same FP64 512^3 kernel ran between 19.2 and 21 ms from one
-//     allocation to the next, against 18.6-19.4 ms row-blocked.
+//     allocation to the next, against 18.6-19.4 ms row-blocked, and 128^3
+//     and 96x128x96 cavities ran 10-14% faster row-blocked.
 //   planes (rs = LX, ps >= N): the reference's [ip][z][y][x] with each plane
-//     staggered by 64 elements.  Used unless LX is a multiple of kRowsLx:
-//     ensemble tiles (up to 256 cells of consecutive x-lines) and the last
-//     block of a line then cover cells of two x-lines, which are contiguous
-//     only in this layout (measured: 48 cubes of side 33-256, FP16, 19.9k
-//     MLUPS in planes against 13.7k row-blocked).
+//     staggered by 64 elements.  Used only for the cavities of an ensemble
+//     whose x-line is not a whole number of 128-byte lines (LX * element
+//     size % 128 != 0): an ensemble's flat tiles let a 64-lane wave cover the
+//     end of one x-line and the start of the next, which are contiguous only
+//     in this layout (row-blocked they would split lines: measured, 48 cubes
+//     of side 33-256, FP16, 19.9k MLUPS in planes against 13.7k).  The single
+//     cavity launches one block per x-line segment, so no wave spans lines.
 //
 // Host copies convert to the reference's order.
-constexpr int kRowsLx = 256;
 struct PopLayout {
     uint64_t ps, rs, elems;  // population stride, row stride, elements of one f+g buffer
 };
-inline PopLayout pop_layout(int lx, int ly, int lz) {
+inline PopLayout pop_layout(int lx, int ly, int lz, size_t elem_bytes, bool flat_tiles) {
     const uint64_t n = static_cast<uint64_t>(lx) * ly * lz;
-    if (lx % kRowsLx == 0) return {static_cast<uint64_t>(lx), 54ull * lx, 54ull * n};
+    if (!flat_tiles || (static_cast<uint64_t>(lx) * elem_bytes) % 128 == 0)
+        return {static_cast<uint64_t>(lx), 54ull * lx, 54ull * n};
     const uint64_t ps = (n + 63) / 64 * 64 + 64;
     return {ps, static_cast<uint64_t>(lx), 54ull * ps};
 }
@@ -204,7 +207,7 @@ struct lbm_ctx {
 
 namespace lbm {
 // Helpers shared by capi.hip and ensemble.hip (defined in capi.hip).
-int ctx_create(const lbm_params *p, hipStream_t stream, lbm_ctx **out);
+int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx **out);
 StepArgs ctx_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, void *gb);
 int ctx_prologue(lbm_ctx *c);    // REF -> FUSED (collides the next step)
 int ctx_ensure_ref(lbm_ctx *c);  // FUSED -> REF (finalize)

@@ -63,8 +63,8 @@ def test_initial_state_bitwise(shape, init):
     assert_state_equal(c, o)
 
 
-# (256, 4, 3) and (256, 5, 3): x-lines of 256 cells select the row-blocked
-# population layout; the other shapes the plane layout.
+# A single cavity uses the row-blocked population layout for every shape;
+# the plane layout (ensemble members) is compared with it in the ensemble tests.
 @pytest.mark.parametrize("shape", [(5, 7, 9), (1, 4, 3), (3, 3, 1), (70, 6, 5), (256, 4, 3)])
 def test_each_stage_bitwise(shape):
     """Every stage entry point against the oracle stage, two full steps."""
