@@ -661,14 +661,12 @@ int lbm_profile(lbm_ctx *c, double *out) {
     if (int rc = check_ctx(c)) return rc;
     if (!out) return fail(LBM_E_INVALID, "out is NULL");
     if (int rc = ensure_ref(c)) return rc;
-    double *dev = nullptr;
-    if (hipMalloc(reinterpret_cast<void **>(&dev), sizeof(double) * 5 * c->p.ly) != hipSuccess)
-        return fail(LBM_E_NOMEM, "profile buffer");
-    hipError_t e = lbm::launch_profile(make_args(c, nullptr, nullptr, nullptr, nullptr), dev, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, sizeof(double) * 5 * c->p.ly, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(dev);
-    if (e != hipSuccess) return hip_fail(e, "profile");
+    // the per-plane results go to the diagnostics partials buffer (at least
+    // 16 * LY doubles), which is free between stream-ordered reductions
+    double *dev = c->partials;
+    LBM_HIP(lbm::launch_profile(make_args(c, nullptr, nullptr, nullptr, nullptr), dev, c->stream));
+    LBM_HIP(hipMemcpyAsync(out, dev, sizeof(double) * 5 * c->p.ly, hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 
