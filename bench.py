@@ -73,6 +73,47 @@ def cpu_baseline(size: int, ra: float, budget_s: float):
     }
 
 
+def run_config1_workload(args) -> dict:
+    """The 64^3 parity case: Ra=1e5, Pr=0.71, walls +-0.5, seed 1, +-1e-3,
+    1000 steps on the GPU (fused path, HIP events) and with the FP64 oracle
+    on the host cores, both timed; relative max-norm and exact-equality
+    counts of f, g, rho', u, T between the two."""
+    import numpy as np
+
+    import oracle
+    from lbm_ddf_amd import Cavity
+
+    L, steps = 64, args.config1_steps
+    params = dict(rayleigh=1.0e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    c = Cavity(L, L, L, **params)
+    c.init_conductive(1, 1.0e-3)
+    c.synchronize()
+    t0 = time.perf_counter()
+    gpu_ms = c.step_timed(steps)
+    c.synchronize()
+    gpu_wall = time.perf_counter() - t0
+    s = c.state()
+    c.close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    o = oracle.OracleState(L, L, L, **params)
+    o.init_conductive(1, 1.0e-3)
+    log(f"config1: GPU {steps} steps in {gpu_wall:.3f} s; oracle on {threads} threads")
+    t0 = time.perf_counter()
+    o.run(steps)
+    cpu_wall = time.perf_counter() - t0
+    cmp = {}
+    for k in ("f", "g", "rho", "ux", "uy", "uz", "phi"):
+        a, b = s[k], o[k]
+        den = float(np.max(np.abs(b)))
+        cmp[k] = {"rel_maxnorm": float(np.max(np.abs(a - b))) / den if den > 0 else float(np.max(np.abs(a - b))),
+                  "differing_values": int(np.count_nonzero(a != b))}
+    n = L ** 3
+    return {"cells": n, "steps": steps,
+            "gpu": {"mlups": n * steps / gpu_wall / 1e6, "wall_s": gpu_wall, "event_ms": gpu_ms},
+            "oracle": {"mlups": n * steps / cpu_wall / 1e6, "wall_s": cpu_wall, "threads": threads},
+            "speedup": cpu_wall / gpu_wall, "compare": cmp}
+
+
 def ensemble_cavities(count: int = 64, draw_seed: int = 4):
     """Config of the 64-cavity ensemble: lateral side from {32,48,64,96,128},
     height from {32,64,128}, Ra log-uniform in [1e4, 1e7], Pr from
@@ -343,9 +384,11 @@ def main() -> int:
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "serve", "precision", "precision-report"],
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "ensemble", "serve", "precision", "precision-report", "config1"],
                     help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--config1-steps", type=int, default=1000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
     ap.add_argument("--sequential-steps", type=int, default=300)
     ap.add_argument("--tuning-cache", action="store_true")
@@ -389,6 +432,23 @@ def main() -> int:
         (ROOT / "gpurun_out").mkdir(exist_ok=True)
         (ROOT / "gpurun_out" / f"precision_{args.precision}_{args.size}.json").write_text(json.dumps(res))
         print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
+        return 0
+
+    if args.workload == "config1":
+        res = run_config1_workload(args)
+        print(json.dumps({
+            "metric": "MLUPS, 64^3 parity case (GPU fused path and FP64 oracle both timed)",
+            "value": round(res["gpu"]["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": res["steps"],
+            "warmup": 0, "ms_per_step": round(res["gpu"]["wall_s"] / res["steps"] * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic: conductive start, seed 1, +-1e-3",
+            "config": {"workload": "Rayleigh-Benard cavity 64x64x64, Ra=1e5, Pr=0.71, walls +-0.5, 1000 steps"},
+            "cpu_baseline": {"value": round(res["oracle"]["mlups"], 3), "unit": "MLUPS",
+                             "cores": res["oracle"]["threads"], "kind": "port",
+                             "sample": f"FP64 oracle, the same 64^3 case, all {res['steps']} steps "
+                                       f"({res['oracle']['wall_s']:.1f} s)"},
+            "config1": res,
+        }), flush=True)
         return 0
 
     if args.workload == "serve":
