@@ -308,8 +308,8 @@ def run_precision_workload(args) -> dict:
     wall = time.perf_counter() - t0
     phi = c.get_field("phi")
     snap = phi[::args.snapshot_stride, ::args.snapshot_stride, ::args.snapshot_stride].astype(np.float64)
-    out_dir = ROOT / "gpurun_out"
-    out_dir.mkdir(exist_ok=True)
+    out_dir = Path(args.out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
     np.save(out_dir / f"precision_T_{prec}_{L}.npy", snap)
     c.close()
     sel = [r for r in recs if r["step"] > args.avg_from]
@@ -326,12 +326,12 @@ def run_precision_workload(args) -> dict:
     return res
 
 
-def precision_report(size: int, avg_from: int) -> dict:
+def precision_report(size: int, avg_from: int, out_dir: str) -> dict:
     """Compares the 16-bit runs of the precision workload with the FP64 run
-    (files written by --workload precision under gpurun_out/)."""
+    (files that --workload precision wrote to out_dir)."""
     import numpy as np
 
-    d = ROOT / "gpurun_out"
+    d = Path(out_dir)
     runs = {}
     for prec in ("fp64", "fp16", "bf16"):
         f = d / f"precision_{prec}_{size}.json"
@@ -405,6 +405,8 @@ def main() -> int:
     ap.add_argument("--sample", type=int, default=100, help="precision workload: steps between Nu samples")
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
+    ap.add_argument("--out-dir", default=str(ROOT / "bench_out"),
+                    help="precision workload: where its series and T snapshots are written and read back")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
                          "the HBM-counter calibration run")
@@ -419,18 +421,18 @@ def main() -> int:
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only; the multi-GPU slab split is a design "
-              "note (DESIGN.md), not an implemented path. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only; the multi-GPU z-slab split is not implemented. "
+              "Run with --gpus 1.", file=sys.stderr)
         return 2
 
     if args.workload == "precision-report":
-        print(json.dumps(precision_report(args.size, args.avg_from), indent=1), flush=True)
+        print(json.dumps(precision_report(args.size, args.avg_from, args.out_dir), indent=1), flush=True)
         return 0
 
     if args.workload == "precision":
         res = run_precision_workload(args)
-        (ROOT / "gpurun_out").mkdir(exist_ok=True)
-        (ROOT / "gpurun_out" / f"precision_{args.precision}_{args.size}.json").write_text(json.dumps(res))
+        Path(args.out_dir).mkdir(parents=True, exist_ok=True)
+        (Path(args.out_dir) / f"precision_{args.precision}_{args.size}.json").write_text(json.dumps(res))
         print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
         return 0
 

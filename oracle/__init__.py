@@ -1,8 +1,9 @@
 """ctypes binding of the FP64 CPU oracle (oracle/oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the correctness checker and the CPU baseline.  Only
-tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
-this package; the product (lbm_ddf_amd) never does.
+TEST INFRASTRUCTURE ONLY: the expected results of the parity tests and the
+CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline legs (timed beside the GPU, and in the 64^3 parity workload compared
+with it) may import this package; the product (lbm_ddf_amd) never does.
 
 Arrays are numpy-owned, shaped like the reference's: populations
 (27, LZ, LY, LX) and fields (LZ, LY, LX), x fastest.

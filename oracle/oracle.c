@@ -1,6 +1,7 @@
 /*
  * oracle.c -- CPU FP64 restatement of LBM_DDF src/ (see oracle.h).
- * TEST INFRASTRUCTURE ONLY: the correctness checker and the CPU baseline.
+ * TEST INFRASTRUCTURE ONLY: the expected results of the parity tests and the
+ * CPU baseline.
  *
  * Build: oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off -fPIC -shared).
  * Cells are processed in parallel with OpenMP where each cell's result

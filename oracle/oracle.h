@@ -2,10 +2,10 @@
  * oracle.h -- CPU FP64 restatement of LBM_DDF's single-GPU D3Q27
  * double-distribution Rayleigh-Benard step (reference tree `src/`).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the correctness checker and the
- * CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it; the product library (lbm_ddf_amd) never
- * links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This library gives the expected results of the
+ * parity tests and the CPU baseline.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's CPU baseline legs may load it; the product library
+ * (lbm_ddf_amd) never links, loads or calls it.
  *
  * It is a plain C transcription written from reading the reference .cu files
  * as text (the reference is CUDA for sm_80 and cannot be built in this image).
