@@ -148,7 +148,9 @@ def run_ensemble_workload(args) -> dict:
                   prandtl=c["prandtl"], t_hot=0.5, t_cold=-0.5)
         e.step(2)
         t = time.perf_counter()
-        e.autotune(8, str(ROOT / "profiles" / "ensemble_tuning.txt") if args.tuning_cache else None)
+        if args.tuning_cache:
+            Path(args.out_dir).mkdir(parents=True, exist_ok=True)
+        e.autotune(8, str(Path(args.out_dir) / "ensemble_tuning.txt") if args.tuning_cache else None)
         bx, bpc, gp, fl, cpt = e.tuning
         e.set_tuning(bx, bpc, 16, fl, cpt)
         log(f"ensemble {prec}: autotune {time.perf_counter() - t:.1f} s -> block {bx}, {bpc} blocks/CU, "
@@ -391,7 +393,8 @@ def main() -> int:
     ap.add_argument("--config1-steps", type=int, default=1000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
     ap.add_argument("--sequential-steps", type=int, default=300)
-    ap.add_argument("--tuning-cache", action="store_true")
+    ap.add_argument("--tuning-cache", action="store_true",
+                    help="ensemble workload: reuse/record the autotuner's choice in <out-dir>/ensemble_tuning.txt")
     ap.add_argument("--staged-steps", type=int, default=20)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-arrivals", type=float, default=60.0, help="seconds during which requests arrive")
@@ -406,7 +409,8 @@ def main() -> int:
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
     ap.add_argument("--out-dir", default=str(ROOT / "bench_out"),
-                    help="precision workload: where its series and T snapshots are written and read back")
+                    help="precision workload: where its series and T snapshots are written and read "
+                         "back; ensemble workload: the tuning cache")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
                          "the HBM-counter calibration run")
