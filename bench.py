@@ -8,8 +8,15 @@ collide-and-stream launch over all cells (f and g).
 Prints one JSON line: MLUPS (whole job), the roofline of the fused kernel
 (algorithmic bytes 2*27*2*8 = 864 B per cell-update in FP64, 216 B in 16-bit
 storage, divided by the kernel's average duration from HIP events recorded on
-the context's stream around the timed launches), and the CPU baseline (the
-FP64 OpenMP oracle on a bounded sample, rank 0 only).
+the context's stream around the timed launches), the CPU baseline (the
+FP64 OpenMP oracle on a bounded sample, rank 0 only) and the same cavity
+with FP16 storage (`fp16_storage`; --no-fp16 skips it).
+
+Other workloads (--workload): ensemble (64 cavities in one launch per step
+against the same cavities one after another), serve (Poisson arrivals
+through the request loop), precision / precision-report (16-bit storage
+against FP64 over a long run), config1 (the 64^3 1000-step parity case with
+GPU and oracle both timed).
 """
 import argparse
 import json
