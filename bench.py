@@ -280,10 +280,14 @@ def batch_means(x, nbatch: int = 10):
     """Mean and 95% half-width from batch means (serially correlated samples)."""
     import numpy as np
 
+    from scipy.stats import t as student_t
+
     x = np.asarray(x, dtype=np.float64)
     b = np.array_split(x, nbatch)
     m = np.array([v.mean() for v in b if len(v)])
-    half = 2.262 * m.std(ddof=1) / np.sqrt(len(m)) if len(m) > 1 else float("nan")  # t(0.975, 9)
+    if len(m) < 2:
+        return float(x.mean()), float("nan")
+    half = student_t.ppf(0.975, len(m) - 1) * m.std(ddof=1) / np.sqrt(len(m))
     return float(x.mean()), float(half)
 
 
