@@ -186,12 +186,18 @@ int run_fused_graph(lbm_ctx *c) {
             c->graph_key[k] = -1;
         }
         hipGraph_t g = nullptr;
+        // capture records the passes without running them; their buffer
+        // bookkeeping is restored if the capture fails part-way
+        const int cf = c->cf, cg = c->cg, layout = c->aa_layout;
         LBM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = LBM_OK;
         for (int i = 0; i < lbm::kGraphPasses && rc == LBM_OK; ++i) rc = run_pass(c, lbm::MODE_FUSED, false);
         const hipError_t ec = hipStreamEndCapture(c->stream, &g);
         if (rc != LBM_OK || ec != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
+            c->cf = cf;
+            c->cg = cg;
+            c->aa_layout = layout;
             return rc != LBM_OK ? rc : hip_fail(ec, "hipStreamEndCapture");
         }
         // the captured passes flipped the bookkeeping an even number of times
