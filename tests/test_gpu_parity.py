@@ -231,3 +231,25 @@ def test_step_staged_reference_structure_bitwise():
     c.step_staged(9)
     o.run(9)
     assert_state_equal(c, o)
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_non_default_physics_bitwise(pattern):
+    """Parameters away from the defaults: gravity 0.5, an explicit beta, the
+    reference's wall temperatures 1 and 0 (so T0 = 0.5 and the lagged force
+    uses phi - 0.5), Pr = 7: fused steps and the reference step structure
+    against the oracle, bit for bit."""
+    params = dict(rayleigh=2.0e5, prandtl=7.0, t_hot=1.0, t_cold=0.0, gravity=0.5, beta=3.0e-3)
+    c = Cavity(23, 9, 7, pattern=pattern, **params)
+    o = oracle.OracleState(23, 9, 7, **params)
+    c.init_conductive(9, 2.0e-2)
+    o.init_conductive(9, 2.0e-2)
+    c.step(13)
+    o.run(13)
+    assert_state_equal(c, o)
+    c.step_staged(3)
+    o.run(3)
+    assert_state_equal(c, o)
+    c.step(70)  # graph-replayed passes
+    o.run(70)
+    assert_state_equal(c, o)
