@@ -1,0 +1,106 @@
+"""The 16-bit population codec, element by element (host copies through the C ABI).
+
+FP16 storage keeps f and g - w_i*T_ref(iy) scaled by 2^15 as IEEE binary16;
+BF16 storage keeps the same deviations as binary32 rounded to nearest even
+to 16 bits.  T_ref(iy) = T_hot - dT*(iy + 1/2)/LY.  The arithmetic is binary32
+without contraction, so numpy's float32/float16 reproduce every stored and
+decoded value exactly: set_pops followed by get_pops must return the numpy
+emulation bit for bit, in both population layouts.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from lbm_ddf_amd import Cavity, Ensemble
+
+pytestmark = pytest.mark.gpu
+
+WALLS = dict(t_hot=0.5, t_cold=-0.5)
+W = oracle.tables()[3]  # reference weights, src/initial.cu:77-83
+
+
+def bf16_round(x32):
+    """binary32 -> bfloat16 bits (round to nearest even, NaN kept quiet)."""
+    u = x32.view(np.uint32).astype(np.uint64)
+    nan = ((u & 0x7F800000) == 0x7F800000) & ((u & 0x007FFFFF) != 0)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, (u >> 16) | 0x40, r)
+    return r.astype(np.uint32)
+
+
+def bf16_to_f32(bits):
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+def t_ref(shape):
+    lz, ly, lx = shape
+    dtref = (WALLS["t_hot"] - WALLS["t_cold"]) / ly
+    tref0 = WALLS["t_hot"] - 0.5 * dtref
+    tr = np.array([tref0 - iy * dtref for iy in range(ly)], dtype=np.float64).astype(np.float32)
+    return tr[None, None, :, None]  # broadcast over (27, z, y, x)
+
+
+def expected(prec, which, values, shape):
+    v = values.astype(np.float32)
+    if which == "g":
+        w = W.astype(np.float32)[:, None, None, None]
+        tr = t_ref(shape)
+        d = (v - w * tr).astype(np.float32)
+    else:
+        d = v
+    if prec == "fp16":
+        h = (d * np.float32(32768.0)).astype(np.float16)
+        back = h.astype(np.float32) * np.float32(1.0 / 32768.0)
+    else:
+        back = bf16_to_f32(bf16_round(d))
+    if which == "g":
+        back = (back + w * tr).astype(np.float32)
+    return back.astype(np.float64)
+
+
+def sample(shape, which, rng):
+    """Deviations over several decades (the codec's working range), signed,
+    on top of w_i*T for g."""
+    z = (27,) + shape
+    mag = 10.0 ** rng.uniform(-7, -0.3, size=z)
+    dev = np.where(rng.random(z) < 0.5, -mag, mag)
+    if which == "f":
+        return dev
+    tr = t_ref(shape).astype(np.float64)
+    return W[:, None, None, None] * tr + dev
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("shape", [(20, 12, 10), (64, 6, 5)])
+def test_codec_roundtrip_is_the_binary16_emulation(prec, shape):
+    c = Cavity(*shape, precision=prec, **WALLS)
+    c.init_conductive(1, 1.0e-3)
+    rng = np.random.default_rng(3)
+    zyx = shape[::-1]
+    for which in ("f", "g"):
+        vals = sample(zyx, which, rng)
+        c.set_pops(which, vals)
+        got = c.get_pops(which)
+        exp = expected(prec, which, vals, zyx)
+        bad = np.argwhere(got != exp)
+        assert bad.size == 0, f"{prec} {which}: {len(bad)} values differ, first {bad[0]}: {got[tuple(bad[0])]!r} vs {exp[tuple(bad[0])]!r}"
+        # and the rounding is within half an ulp of the format
+        dev = (vals - (W[:, None, None, None] * t_ref(zyx) if which == "g" else 0.0))
+        rel = 2.0 ** -11 if prec == "fp16" else 2.0 ** -8
+        assert np.all(np.abs(got - vals) <= rel * np.abs(dev) + 1e-7 * np.abs(vals) + 2.0 ** -40)
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_codec_roundtrip_plane_layout_ensemble_member(prec):
+    """An ensemble member with a 20-cell x-line uses population planes."""
+    e = Ensemble(prec)
+    slot = e.add(20, 9, 7, seed=2, amp=1.0e-3, rayleigh=1e5, prandtl=0.71, **WALLS)
+    m = e.cavity(slot)
+    rng = np.random.default_rng(4)
+    zyx = (7, 9, 20)
+    for which in ("f", "g"):
+        vals = sample(zyx, which, rng)
+        m.set_pops(which, vals)
+        np.testing.assert_array_equal(m.get_pops(which), expected(prec, which, vals, zyx))
+    e.close()
