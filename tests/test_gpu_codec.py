@@ -104,3 +104,41 @@ def test_codec_roundtrip_plane_layout_ensemble_member(prec):
         m.set_pops(which, vals)
         np.testing.assert_array_equal(m.get_pops(which), expected(prec, which, vals, zyx))
     e.close()
+
+
+@pytest.mark.parametrize("prec,rel", [("fp16", 2.0 ** -11), ("bf16", 2.0 ** -8)])
+@pytest.mark.parametrize("nsteps,roundings", [(1, 2), (2, 4)])
+def test_16bit_step_is_the_fp64_step_within_its_roundings(prec, rel, nsteps, roundings):
+    """From the same stored state and fields, the 16-bit path (binary32
+    arithmetic) and the FP64 oracle agree cell by cell to within the storage
+    roundings on the way: one step stores each post-collision value (with
+    its source row's reference temperature) and the finalize stores it again
+    (with its own row's), two steps add the fused pass's pair.  Each rounding
+    is at most half an ulp of the stored deviation, which differs between
+    the two rows by w_i * dT / LY; binary32 arithmetic adds ~1e-7.  A wrong
+    row temperature, weight or wall rule errs by w_i * dT / LY (>= 4e-4
+    here), far above the bound."""
+    shape = (20, 12, 10)
+    zyx = shape[::-1]
+    c = Cavity(*shape, precision=prec, rayleigh=1e6, prandtl=0.71, **WALLS)
+    o = oracle.OracleState(*shape, rayleigh=1e6, prandtl=0.71, **WALLS)
+    c.init_conductive(1, 1.0e-2)
+    o.init_conductive(1, 1.0e-2)
+    for k in ("rho", "ux", "uy", "uz", "phi", "fy"):
+        np.testing.assert_array_equal(c.get_field(k), o[k])
+    # the oracle starts from exactly the stored (rounded) populations
+    o["f"] = c.get_pops("f")
+    o["g"] = c.get_pops("g")
+    c.step(nsteps)
+    o.run(nsteps)
+    dtref = (WALLS["t_hot"] - WALLS["t_cold"]) / shape[1]
+    w = W[:, None, None, None]
+    tr = t_ref(zyx).astype(np.float64)
+    for which, dev, shift in (("f", o["f"], 0.0), ("g", o["g"] - w * tr, 2.0 * w * dtref)):
+        got, exp = c.get_pops(which), o[which]
+        bound = roundings * rel * (np.abs(dev) + shift) + 2e-7
+        err = np.abs(got - exp)
+        worst = np.unravel_index(np.argmax(err - bound), err.shape)
+        assert np.all(err <= bound), f"{prec} {which}: error {err[worst]:.3e} > bound {bound[worst]:.3e} at {worst}"
+    # phi is the FP32 sum of the decoded g: the same roundings, summed
+    assert np.max(np.abs(c.get_field("phi") - o["phi"])) < 27 * roundings * rel * 0.02 + 1e-6
