@@ -148,7 +148,9 @@ def _check(rc: int) -> None:
 
 
 def _dptr(a: np.ndarray):
-    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    # checked always (not an assert): the library writes through this pointer
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(_D)
 
 
