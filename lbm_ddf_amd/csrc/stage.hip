@@ -308,24 +308,41 @@ __device__ __forceinline__ double codec_tref(uint64_t i, const CodecArgs &c) {
     return c.tref0 - static_cast<double>(iy) * c.dtref;
 }
 
+// Host copies convert in binary64: a decoded value is the stored value
+// exactly (the stored deviation, unscaled, plus w_i * T_ref(iy) for g), and
+// encoding a decoded value gives back the same bits, so reads, writes and
+// checkpoints round-trip.  A new value is rounded to nearest through binary32
+// (the formats' own rounding, as in the step kernels).
+template <class P>
+__device__ __forceinline__ double stored_deviation(typename P::S s) {
+    if constexpr (std::is_same_v<P, PolicyF64>) return s;
+    else if constexpr (std::is_same_v<P, PolicyF16>) return static_cast<double>(static_cast<float>(s)) * (1.0 / 32768.0);
+    else return static_cast<double>(bf16_bits_to_float(s));
+}
+template <class P>
+__device__ __forceinline__ typename P::S store_deviation(double d) {
+    if constexpr (std::is_same_v<P, PolicyF64>) return d;
+    else if constexpr (std::is_same_v<P, PolicyF16>) return static_cast<_Float16>(static_cast<float>(d * 32768.0));
+    else return float_to_bf16_bits(static_cast<float>(d));
+}
+
 template <class P, int WHICH, int IP>
 __global__ void decode_kernel(const void *src, double *dst, uint64_t n, CodecArgs c) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const auto s = static_cast<const typename P::S *>(src)[slot_of(i, c.lx, c.rs)];
-    if constexpr (WHICH == 0) dst[i] = static_cast<double>(P::dec_f(s));
-    else dst[i] = static_cast<double>(P::template dec_g<IP>(s, static_cast<typename P::T>(codec_tref(i, c))));
+    if constexpr (WHICH == 0 || P::exact) dst[i] = stored_deviation<P>(s);
+    else dst[i] = stored_deviation<P>(s) + weight(IP) * codec_tref(i, c);
 }
 
 template <class P, int WHICH, int IP>
 __global__ void encode_kernel(const double *src, void *dst, uint64_t n, CodecArgs c) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const auto v = static_cast<typename P::T>(src[i]);
     auto *d = static_cast<typename P::S *>(dst);
     const uint64_t at = slot_of(i, c.lx, c.rs);
-    if constexpr (WHICH == 0) d[at] = P::enc_f(v);
-    else d[at] = P::template enc_g<IP>(v, static_cast<typename P::T>(codec_tref(i, c)));
+    if constexpr (WHICH == 0 || P::exact) d[at] = store_deviation<P>(src[i]);
+    else d[at] = store_deviation<P>(src[i] - weight(IP) * codec_tref(i, c));
 }
 
 template <class P>

@@ -25,7 +25,7 @@ def states_equal(a, b):
         assert np.array_equal(sa[k], sb[k]), k
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp16"])
+@pytest.mark.parametrize("precision", ["fp64", "fp16", "bf16"])
 def test_restart_continues_bit_for_bit(tmp_path, precision):
     shape = (14, 10, 9)
     a = Cavity(*shape, precision=precision, **RB)
