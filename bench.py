@@ -244,10 +244,11 @@ def run_serve_workload(args) -> dict:
     reqs = serve_requests(args.serve_rate, args.serve_arrivals)
     cells = [r["lx"] ** 3 for r in reqs]
     log(f"serve: {len(reqs)} requests over {args.serve_arrivals:.0f} s, mean {sum(cells) / max(1, len(cells)) / 1e6:.2f} M "
-        f"cells, precision {args.serve_precision}, time limit {args.serve_time_limit:.0f} s")
+        f"cells, precision {args.serve_precision}, pattern {args.serve_pattern}, "
+        f"time limit {args.serve_time_limit:.0f} s")
     res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
                     hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit,
-                    max_resident_cells=int(args.serve_max_cells), progress_s=30.0)
+                    max_resident_cells=int(args.serve_max_cells), progress_s=30.0, pattern=args.serve_pattern)
     log(f"serve: {st['completed']} completed, {st['unfinished']} unfinished, {st['pending']} pending, "
         f"{st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
         f"p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; peak resident {st['peak_resident']}")
@@ -255,7 +256,7 @@ def run_serve_workload(args) -> dict:
     tot, upd = 0.0, 0.0
     for r in reqs[: args.serve_seq_sample]:
         c = Cavity(r["lx"], r["ly"], r["lz"], rayleigh=r["rayleigh"], prandtl=r["prandtl"], t_hot=0.5, t_cold=-0.5,
-                   precision=args.serve_precision)
+                   precision=args.serve_precision, pattern=args.serve_pattern)
         c.init_conductive(r["seed"], r["amp"])
         c.step(4)
         c.synchronize()
@@ -411,6 +412,8 @@ def main() -> int:
     ap.add_argument("--serve-arrivals", type=float, default=60.0, help="seconds during which requests arrive")
     ap.add_argument("--serve-time-limit", type=float, default=120.0)
     ap.add_argument("--serve-precision", default="fp16", choices=["fp64", "fp16", "bf16"])
+    ap.add_argument("--serve-pattern", default="ab", choices=["ab", "aa"],
+                    help="resident cavities with two population buffers or one, updated in place")
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     ap.add_argument("--serve-seq-sample", type=int, default=16)
@@ -479,7 +482,8 @@ def main() -> int:
             "data": f"synthetic: Poisson arrivals {args.serve_rate:g}/s for {args.serve_arrivals:g} s, seed 5",
             "config": {"workload": "RB cavities, side log-U[32,256], Ra log-U[1e4,1e8], Pr {0.71,1,7}, "
                                    "stop |dNu|<1e-3 per 1000 steps or 50k steps, HBM budget "
-                                   f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit},
+                                   f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit,
+                       "pattern": args.serve_pattern},
             "p50_s": st["p50_s"], "p99_s": st["p99_s"], "serve": res,
         }), flush=True)
         return 0

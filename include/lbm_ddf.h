@@ -220,7 +220,10 @@ typedef struct lbm_ensemble lbm_ensemble;
 int lbm_ensemble_create(int32_t precision, lbm_ensemble **out);
 int lbm_ensemble_destroy(lbm_ensemble *ens);
 /* Allocates a cavity (params->precision must match) with the conductive
- * start of lbm_init_conductive; *slot receives its index. */
+ * start of lbm_init_conductive; *slot receives its index.  params->pattern
+ * must match the resident members' (the first cavity of an empty ensemble
+ * sets it): in-place members need half the population memory, so more of
+ * them fit a budget, with the same results bit for bit. */
 int lbm_ensemble_add(lbm_ensemble *ens, const lbm_params *params, uint64_t seed, double amp, int32_t *slot);
 int lbm_ensemble_remove(lbm_ensemble *ens, int32_t slot);
 int lbm_ensemble_count(const lbm_ensemble *ens, int32_t *slots, int32_t *resident, uint64_t *cells);
@@ -301,6 +304,9 @@ typedef struct lbm_serve_options {
     uint64_t max_resident_cells; /* 0: no cap; else admit while resident cells stay below it
                                     (enough cells to saturate HBM, so admitted requests finish) */
     double progress_s;           /* > 0: the loop prints a progress line to stderr at this interval */
+    int32_t pattern;             /* LBM_PATTERN_AB (0) or LBM_PATTERN_AA: in place, half the
+                                    population bytes per request, same results */
+    int32_t reserved;            /* must be 0 */
 } lbm_serve_options;
 
 typedef struct lbm_serve_stats {

@@ -474,7 +474,8 @@ class ServeOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("chunk_steps", ctypes.c_int64),
                 ("hbm_budget", ctypes.c_uint64), ("time_limit_s", ctypes.c_double), ("block_x", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("flags", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
-                ("max_resident_cells", ctypes.c_uint64), ("progress_s", ctypes.c_double)]
+                ("max_resident_cells", ctypes.c_uint64), ("progress_s", ctypes.c_double),
+                ("pattern", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ServeStats(ctypes.Structure):
@@ -495,11 +496,13 @@ SIGNATURES["lbm_serve"] = (ctypes.c_int, [ctypes.POINTER(Request), ctypes.c_int3
 
 def serve(requests: list, precision: str = "fp64", chunk_steps: int = 100, hbm_budget: float = 250e9,
           time_limit_s: float = 60.0, max_resident: int = 0, block_x: int = 0, blocks_per_cu: int = 8,
-          flags: int = 7, max_resident_cells: int = 0, cells_per_thread: int = 1, progress_s: float = 0.0):
+          flags: int = 7, max_resident_cells: int = 0, cells_per_thread: int = 1, progress_s: float = 0.0,
+          pattern: str = "ab"):
     """Run lbm_serve over a list of request dicts (keys of Request; defaults:
     walls +-0.5, amp 1e-3).  Returns (results list of dicts, stats dict).
     progress_s > 0 makes the loop print a progress line to stderr at that
-    interval."""
+    interval; pattern "aa" keeps every resident cavity in place (half the
+    population bytes, same results)."""
     lib = load()
     n = len(requests)
     arr = (Request * max(1, n))()
@@ -510,7 +513,7 @@ def serve(requests: list, precision: str = "fp64", chunk_steps: int = 100, hbm_b
             setattr(arr[i], k, q[k])
     opt = ServeOptions(PREC[precision], int(max_resident), int(chunk_steps), int(hbm_budget), float(time_limit_s),
                        int(block_x), int(blocks_per_cu), int(flags), int(cells_per_thread), int(max_resident_cells),
-                       float(progress_s))
+                       float(progress_s), PATTERN[pattern], 0)
     res = (Result * max(1, n))()
     st = ServeStats()
     _check(lib.lbm_serve(arr, n, ctypes.byref(opt), res, ctypes.byref(st)))

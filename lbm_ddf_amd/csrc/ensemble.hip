@@ -36,11 +36,11 @@
 namespace lbm {
 
 hipError_t launch_ens_lp(int prec, const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity,
-                         bool reduce, int bx, int flags, int grid, hipStream_t s);
+                         bool reduce, bool aa, int bx, int flags, int grid, hipStream_t s);
 
 hipError_t launch_ens_f64(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
-                          int bx, int flags, int grid, hipStream_t s) {
-    return launch_prec<PolicyF64>(cav, ctrl, tmap, parity, reduce, bx, flags, grid, s);
+                          bool aa, int bx, int flags, int grid, hipStream_t s) {
+    return launch_prec<PolicyF64>(cav, ctrl, tmap, parity, reduce, aa, bx, flags, grid, s);
 }
 
 namespace {
@@ -54,11 +54,11 @@ __global__ __launch_bounds__(kFinishThreads) void ens_finish_kernel(const EnsCav
 }
 
 hipError_t launch_ens(int prec, const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity,
-                      bool reduce, int bx, int flags, int grid, hipStream_t s) {
+                      bool reduce, bool aa, int bx, int flags, int grid, hipStream_t s) {
     switch (prec) {
-        case LBM_PREC_FP64: return launch_ens_f64(cav, ctrl, tmap, parity, reduce, bx, flags, grid, s);
+        case LBM_PREC_FP64: return launch_ens_f64(cav, ctrl, tmap, parity, reduce, aa, bx, flags, grid, s);
         case LBM_PREC_FP16:
-        case LBM_PREC_BF16: return launch_ens_lp(prec, cav, ctrl, tmap, parity, reduce, bx, flags, grid, s);
+        case LBM_PREC_BF16: return launch_ens_lp(prec, cav, ctrl, tmap, parity, reduce, aa, bx, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -68,6 +68,9 @@ hipError_t launch_ens(int prec, const EnsCavity *cav, const uint32_t *ctrl, cons
 
 struct lbm_ensemble {
     int prec = 0;
+    // storage pattern of the members: set by the first cavity added to an
+    // empty ensemble; every resident member shares it
+    bool aa = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<lbm_ctx *> slots;  // nullptr = free slot
@@ -159,8 +162,14 @@ int upload_tables(lbm_ensemble *e) {
         if (!c) continue;
         lbm::EnsCavity &d = e->cav_host[ncav];
         std::memset(&d, 0, sizeof(d));
-        d.a[0] = lbm::ctx_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
-        d.a[1] = lbm::ctx_args(c, c->f[1 - c->cf], c->g[1 - c->cg], c->f[c->cf], c->g[c->cg]);
+        if (c->aa) {
+            // one buffer, written in place; the pass follows its layout
+            d.a[0] = d.a[1] = lbm::ctx_args(c, nullptr, nullptr, c->f[0], c->g[0]);
+            d.aa_odd0 = c->aa_layout == lbm::AA_LAYOUT_E ? 1 : 0;
+        } else {
+            d.a[0] = lbm::ctx_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
+            d.a[1] = lbm::ctx_args(c, c->f[1 - c->cf], c->g[1 - c->cg], c->f[c->cf], c->g[c->cg]);
+        }
         d.d.lx = c->p.lx;
         d.d.ly = c->p.ly;
         d.d.lz = c->p.lz;
@@ -218,7 +227,7 @@ int prologue_all(lbm_ensemble *e) {
 }
 
 int launch_one(lbm_ensemble *e, int parity, bool reduce) {
-    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, parity, reduce, e->bx, e->flags, grid_of(e),
+    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, parity, reduce, e->aa, e->bx, e->flags, grid_of(e),
                             e->stream));
     return LBM_OK;
 }
@@ -228,9 +237,9 @@ int build_graph(lbm_ensemble *e) {
     hipGraph_t g = nullptr;
     ENS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < e->graph_pairs; ++k) {
-        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 0, false, e->bx, e->flags,
+        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 0, false, e->aa, e->bx, e->flags,
                                         grid_of(e), e->stream);
-        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 1, false, e->bx, e->flags,
+        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 1, false, e->aa, e->bx, e->flags,
                                         grid_of(e), e->stream);
         if (e0 != hipSuccess || e1 != hipSuccess) {
             (void)hipStreamEndCapture(e->stream, &g);
@@ -277,8 +286,12 @@ int run_fused(lbm_ensemble *e, int64_t nsteps, bool reduce_last) {
     for (lbm_ctx *c : e->slots) {
         if (!c || c->phase != lbm::PHASE_FUSED) continue;
         if (nsteps & 1) {
-            c->cf = 1 - c->cf;
-            c->cg = 1 - c->cg;
+            if (c->aa) {
+                c->aa_layout = c->aa_layout == lbm::AA_LAYOUT_E ? lbm::AA_LAYOUT_N : lbm::AA_LAYOUT_E;
+            } else {
+                c->cf = 1 - c->cf;
+                c->cg = 1 - c->cg;
+            }
         }
         c->nstep += nsteps;
     }
@@ -352,8 +365,11 @@ int lbm_ensemble_destroy(lbm_ensemble *e) {
 int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double amp, int32_t *slot) {
     if (!e || !p || !slot) return ctx_fail(LBM_E_INVALID, "NULL argument");
     if (p->precision != e->prec) return ctx_fail(LBM_E_INVALID, "cavity precision differs from the ensemble's");
-    if (p->pattern != LBM_PATTERN_AB)
-        return ctx_fail(LBM_E_UNSUPPORTED, "ensemble members use the A/B pattern");
+    const bool aa = p->pattern == LBM_PATTERN_AA;
+    bool empty = true;
+    for (const lbm_ctx *m : e->slots) empty = empty && m == nullptr;
+    if (!empty && aa != e->aa)
+        return ctx_fail(LBM_E_INVALID, "cavity storage pattern differs from the resident members'");
     lbm_ctx *c = nullptr;
     if (int rc = lbm::ctx_create(p, e->stream, true, &c)) return rc;
     if (int rc = lbm_init_conductive(c, seed, amp)) {
@@ -365,6 +381,8 @@ int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double
     if (s == e->slots.size()) e->slots.push_back(c);
     else e->slots[s] = c;
     *slot = static_cast<int32_t>(s);
+    if (aa != e->aa) drop_graph(e);  // the graph holds the other pattern's kernel
+    e->aa = aa;
     e->tables_dirty = true;
     return LBM_OK;
 }

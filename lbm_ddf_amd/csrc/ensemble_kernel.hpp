@@ -3,19 +3,28 @@
 // (FP64, no FMA contraction) and ensemble_lp.hip (16-bit storage).
 #pragma once
 
-#include "step_kernel.hpp"
+#include "aa_kernel.hpp"
 
 namespace lbm {
 
 struct EnsCavity {
-    StepArgs a[2];  // [0]: current buffers -> other; [1]: other -> current
+    // A/B pattern: [0] current buffers -> other, [1] other -> current.
+    // In-place pattern: both the one buffer.
+    StepArgs a[2];
     DiagArgs d;
     uint32_t tile0, ntiles;
     int32_t active;
     int32_t slot;
+    // In-place pattern: 1 when the cavity's pass at launch parity 0 is the
+    // odd one (its buffer holds the exchanged layout), else 0.
+    int32_t aa_odd0;
+    int32_t pad;
 };
 
-template <class P, int MODE, bool REDUCE, int BX, int FLAGS>
+// One launch of the ensemble step: every resident cavity advances by one
+// pass.  AA selects the in-place passes (even or odd per cavity, from
+// parity and aa_odd0); otherwise the A/B pass from buffer set `parity`.
+template <class P, int MODE, bool REDUCE, int BX, int FLAGS, bool AA>
 __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restrict__ cav,
                                                       const uint32_t *__restrict__ ctrl,
                                                       const uint32_t *__restrict__ tmap, int parity) {
@@ -47,7 +56,18 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
         ci_next = t + t_step < t_end ? tmap[t + t_step] : 0u;
         const EnsCavity &c = cav[ci];
         if (!c.active) continue;
-        const StepArgs &a = c.a[parity];
+        const StepArgs &a = c.a[AA ? 0 : parity];
+        const bool odd = AA && (parity ^ c.aa_odd0) != 0;
+        // the cell update of this cavity's pass
+        auto cell_update = [&](int ix, int iy, int iz, double(&qc)[NQ]) {
+            constexpr int FL = FLAGS & (kNtStore | kNtLoad);
+            if constexpr (AA) {
+                if (odd) aa_cell<P, MODE_AA_ODD, REDUCE, FL>(a, ix, iy, iz, qc);
+                else aa_cell<P, MODE_AA_EVEN, REDUCE, FL>(a, ix, iy, iz, qc);
+            } else {
+                step_cell<P, MODE, REDUCE, FL>(a, ix, iy, iz, qc);
+            }
+        };
         const uint32_t lt = t - c.tile0;
         double q[NQ];
         if constexpr (REDUCE) {
@@ -64,53 +84,60 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
             const int iz = static_cast<int>(r / ly);
             if constexpr (REDUCE) {
                 double qc[NQ];
-                step_cell<P, MODE, true, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, qc);
+                cell_update(ix, iy, iz, qc);
 #pragma unroll
                 for (int j = 0; j < NQ; ++j) q[j] = (j == Q_MAXUSQ) ? fmax(q[j], qc[j]) : q[j] + qc[j];
             } else {
-                step_cell<P, MODE, false, FLAGS & (kNtStore | kNtLoad)>(a, ix, iy, iz, q);
+                cell_update(ix, iy, iz, q);
             }
         }
         if constexpr (REDUCE) block_reduce_diag<BX>(q, a.partials + static_cast<uint64_t>(lt) * NQ);
     }
 }
 
-template <class P, int BX, int FL>
+template <class P, int BX, int FL, bool AA>
 hipError_t launch_fl(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, int grid,
                      hipStream_t s) {
-    hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX, FL>), dim3(grid), dim3(BX), 0, s, cav, ctrl, tmap,
-                       parity);
+    hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, false, BX, FL, AA>), dim3(grid), dim3(BX), 0, s, cav, ctrl,
+                       tmap, parity);
     return hipGetLastError();
 }
 
 // Flags of the plain launch: 0, 3 (non-temporal), 4 (XCD order), 7 (both).
 // The launch that folds the diagnostics uses the XCD order only.
-template <class P, int BX>
+template <class P, int BX, bool AA>
 hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
                      int flags, int grid, hipStream_t s) {
     if (reduce) {
-        hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX, kXcdSwizzle>), dim3(grid), dim3(BX), 0, s, cav,
-                           ctrl, tmap, parity);
+        hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX, kXcdSwizzle, AA>), dim3(grid), dim3(BX), 0, s,
+                           cav, ctrl, tmap, parity);
         return hipGetLastError();
     }
     switch (flags) {
-        case 0: return launch_fl<P, BX, 0>(cav, ctrl, tmap, parity, grid, s);
-        case 3: return launch_fl<P, BX, 3>(cav, ctrl, tmap, parity, grid, s);
-        case 4: return launch_fl<P, BX, 4>(cav, ctrl, tmap, parity, grid, s);
-        case 7: return launch_fl<P, BX, 7>(cav, ctrl, tmap, parity, grid, s);
+        case 0: return launch_fl<P, BX, 0, AA>(cav, ctrl, tmap, parity, grid, s);
+        case 3: return launch_fl<P, BX, 3, AA>(cav, ctrl, tmap, parity, grid, s);
+        case 4: return launch_fl<P, BX, 4, AA>(cav, ctrl, tmap, parity, grid, s);
+        case 7: return launch_fl<P, BX, 7, AA>(cav, ctrl, tmap, parity, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class P, bool AA>
+hipError_t launch_pattern(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
+                          int bx, int flags, int grid, hipStream_t s) {
+    switch (bx) {
+        case 64: return launch_bx<P, 64, AA>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
+        case 128: return launch_bx<P, 128, AA>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
+        case 256: return launch_bx<P, 256, AA>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class P>
 hipError_t launch_prec(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
-                       int bx, int flags, int grid, hipStream_t s) {
-    switch (bx) {
-        case 64: return launch_bx<P, 64>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
-        case 128: return launch_bx<P, 128>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
-        case 256: return launch_bx<P, 256>(cav, ctrl, tmap, parity, reduce, flags, grid, s);
-        default: return hipErrorInvalidValue;
-    }
+                       bool aa, int bx, int flags, int grid, hipStream_t s) {
+    return aa ? launch_pattern<P, true>(cav, ctrl, tmap, parity, reduce, bx, flags, grid, s)
+              : launch_pattern<P, false>(cav, ctrl, tmap, parity, reduce, bx, flags, grid, s);
 }
 
 }  // namespace lbm

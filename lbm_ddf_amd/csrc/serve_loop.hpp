@@ -56,6 +56,8 @@ inline const char *serve_args_error(const lbm_request *reqs, int32_t n, const lb
                                     const lbm_result *results) {
     if (!reqs || n < 0 || !opt || !results) return "NULL argument";
     if (opt->chunk_steps < 1) return "chunk_steps must be >= 1";
+    if (opt->pattern != LBM_PATTERN_AB && opt->pattern != LBM_PATTERN_AA) return "unknown pattern";
+    if (opt->reserved != 0) return "reserved must be 0";
     for (int32_t i = 0; i < n; ++i)
         if (reqs[i].window < opt->chunk_steps || reqs[i].window % opt->chunk_steps != 0 || reqs[i].max_steps < 1)
             return "each window must be a positive multiple of chunk_steps";

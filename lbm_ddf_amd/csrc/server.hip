@@ -1,7 +1,7 @@
 // server.hip -- lbm_serve: the continuous-batching request loop of
 // serve_loop.hpp over an ensemble (one persistent launch per step for all
-// resident cavities, diagnostics folded into each chunk's last launch) and
-// the steady clock.
+// resident cavities, diagnostics folded into each chunk's last launch; A/B or
+// in-place members) and the steady clock.
 #include <chrono>
 #include <thread>
 #include <vector>
@@ -15,6 +15,7 @@ namespace {
 struct EnsembleBackend {
     lbm_ensemble *ens;
     int32_t precision;
+    int32_t pattern;
 
     lbm_params params(const lbm_request &q) const {
         lbm_params p{};
@@ -28,6 +29,7 @@ struct EnsembleBackend {
         p.t_cold = q.t_cold;
         p.gravity = 1.0;
         p.beta = 0.0;
+        p.pattern = pattern;
         return p;
     }
     int need(const lbm_request &q, uint64_t *bytes) const {
@@ -76,7 +78,7 @@ int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, 
             lbm_ensemble_destroy(ens);
             return rc;
         }
-    EnsembleBackend be{ens, opt->precision};
+    EnsembleBackend be{ens, opt->precision, opt->pattern};
     SteadyClock clk;
     const int rc = lbm::serve_loop(be, clk, reqs, n, opt, results, stats);
     lbm_ensemble_destroy(ens);

@@ -28,24 +28,26 @@ WALLS = dict(t_hot=0.5, t_cold=-0.5)
 AMP = 1.0e-2
 
 
-def standalone(cav, nsteps, precision="fp64"):
-    c = Cavity(*cav["shape"], rayleigh=cav["rayleigh"], prandtl=cav["prandtl"], precision=precision, **WALLS)
+def standalone(cav, nsteps, precision="fp64", pattern="ab"):
+    c = Cavity(*cav["shape"], rayleigh=cav["rayleigh"], prandtl=cav["prandtl"], precision=precision, pattern=pattern,
+               **WALLS)
     c.init_conductive(cav["seed"], AMP)
     c.step(nsteps)
     return c
 
 
-def build(precision="fp64", cavs=CAVS):
+def build(precision="fp64", cavs=CAVS, pattern="ab"):
     e = Ensemble(precision)
-    slots = [e.add(*c["shape"], seed=c["seed"], amp=AMP, rayleigh=c["rayleigh"], prandtl=c["prandtl"], **WALLS)
+    slots = [e.add(*c["shape"], seed=c["seed"], amp=AMP, rayleigh=c["rayleigh"], prandtl=c["prandtl"],
+                   pattern=pattern, **WALLS)
              for c in cavs]
     return e, slots
 
 
-def assert_member_equals(e, slot, cav, precision="fp64"):
+def assert_member_equals(e, slot, cav, precision="fp64", pattern="ab"):
     m = e.cavity(slot)
     k = m.step_count
-    ref = standalone(cav, k, precision)
+    ref = standalone(cav, k, precision, pattern)
     a, b = m.state(), ref.state()
     for key in a:
         assert np.array_equal(a[key], b[key]), f"slot {slot} {key} differs after {k} steps"
@@ -159,4 +161,69 @@ def test_injected_nan_stays_in_its_cavity():
             continue
         assert diags[s]["nonfinite"] == 0
         assert_member_equals(e, s, cav)
+    e.close()
+
+
+@pytest.mark.parametrize("graph_pairs,cpt", [(0, 1), (3, 1), (3, 4)])
+def test_in_place_members_equal_standalone_runs(graph_pairs, cpt):
+    """In-place members (one buffer, even and odd passes per cavity, from its
+    own layout) equal standalone runs of either pattern bit for bit, across
+    odd step counts, graph replays and host reads between calls."""
+    e, slots = build(pattern="aa")
+    e.set_tuning(128, 4, graph_pairs, 7 if graph_pairs else 0, cpt)
+    total = 0
+    for n in (13, 14):
+        e.step(n)
+        total += n + 1  # reads below finalize the members: a prologue per call
+        for s, cav in zip(slots, CAVS):
+            assert assert_member_equals(e, s, cav, pattern="aa") == total
+            assert assert_member_equals(e, s, cav, pattern="ab") == total
+    e.close()
+
+
+def test_in_place_members_join_leave_and_diag():
+    """Members at different step parities share the launches: a cavity that
+    joins later runs its even pass while the others run their odd one."""
+    e, slots = build(cavs=CAVS[:3], pattern="aa")
+    e.set_tuning(64, 4, 2, 7)
+    e.step(9)
+    s4 = e.add(*CAVS[3]["shape"], seed=CAVS[3]["seed"], amp=AMP, rayleigh=CAVS[3]["rayleigh"],
+               prandtl=CAVS[3]["prandtl"], pattern="aa", **WALLS)
+    e.step(8)
+    e.remove(slots[1])
+    diags = e.run_diag(5)
+    assert assert_member_equals(e, slots[0], CAVS[0], pattern="aa") == 23  # prologue + 9 + 8 + 5
+    assert assert_member_equals(e, s4, CAVS[3], pattern="aa") == 14  # prologue + 8 + 5
+    for s, cav in ((slots[0], CAVS[0]), (s4, CAVS[3])):
+        d = diags[s]
+        r = standalone(cav, d["step"], pattern="aa").reduce()
+        for k in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy", "mass"):
+            assert np.isclose(d[k], r[k], rtol=1e-11, atol=1e-14), (s, k, d[k], r[k])
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_in_place_reduced_precision_members(precision):
+    """16-bit in place: equal to the standalone in-place run bit for bit (the
+    A/B run rounds g against other rows' reference temperatures, so it is
+    close, not equal)."""
+    e, slots = build(precision, pattern="aa")
+    e.step(21)
+    for s, cav in zip(slots, CAVS):
+        assert_member_equals(e, s, cav, precision, pattern="aa")
+    e.close()
+
+
+def test_members_share_one_storage_pattern():
+    e, slots = build(cavs=CAVS[:2])
+    c = CAVS[2]
+    with pytest.raises(RuntimeError, match="pattern"):
+        e.add(*c["shape"], seed=c["seed"], amp=AMP, rayleigh=c["rayleigh"], prandtl=c["prandtl"], pattern="aa",
+              **WALLS)
+    for s in slots:
+        e.remove(s)
+    s = e.add(*c["shape"], seed=c["seed"], amp=AMP, rayleigh=c["rayleigh"], prandtl=c["prandtl"], pattern="aa",
+              **WALLS)  # an empty ensemble takes the new pattern
+    e.step(6)
+    assert assert_member_equals(e, s, c, pattern="aa") == 7
     e.close()

@@ -98,3 +98,38 @@ def test_fp16_request_outside_the_codec_range_is_rejected_not_run():
     assert res[0]["status"] == "rejected"
     assert res[1]["status"] == "budget" and res[1]["steps"] == 300
     assert st["rejected"] == 1 and st["completed"] == 1
+
+
+def test_in_place_serving_gives_the_same_results():
+    """FP64 cavities kept in place (one population buffer) give every request
+    exactly the A/B result."""
+    mix = [dict(B, arrival_s=0.0), dict(A, arrival_s=0.05), dict(C, arrival_s=0.0), dict(D, arrival_s=0.1)]
+    ab, _ = serve(mix, chunk_steps=50)
+    aa, _ = serve(mix, chunk_steps=50, pattern="aa")
+    for r1, r2 in zip(ab, aa):
+        same(r1, r2)
+
+
+def test_in_place_fp16_result_same_alone_and_in_mix():
+    """16-bit in place rounds g against other rows than A/B does, so its
+    results are its own -- but they do not depend on the mix."""
+    alone, _ = serve([A], precision="fp16", chunk_steps=50, pattern="aa")
+    mix = [dict(B, arrival_s=0.0), dict(A, arrival_s=0.05), dict(C, arrival_s=0.0), dict(D, arrival_s=0.1)]
+    res, st = serve(mix, precision="fp16", chunk_steps=50, pattern="aa")
+    same(alone[0], res[1])
+    assert st["completed"] == 4
+
+
+def test_in_place_serving_admits_more_under_one_budget():
+    need_ab = bytes_required(make_params(24, 24, 24))
+    need_aa = bytes_required(make_params(24, 24, 24, pattern="aa"))
+    assert need_aa < 0.6 * need_ab
+    reqs = [req(24, 24, 24, 1e5, s, max_steps=200) for s in range(6)]
+    budget = 2 * need_ab + need_ab // 2  # two A/B cavities fit, four in place
+    res_ab, st_ab = serve(reqs, chunk_steps=50, hbm_budget=budget)
+    res_aa, st_aa = serve(reqs, chunk_steps=50, hbm_budget=budget, pattern="aa")
+    assert st_ab["peak_resident"] == 2
+    assert st_aa["peak_resident"] >= 4
+    assert st_aa["completed"] == st_ab["completed"] == 6
+    for r1, r2 in zip(res_ab, res_aa):
+        same(r1, r2)
