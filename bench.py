@@ -152,7 +152,7 @@ def run_ensemble_workload(args) -> dict:
         e = Ensemble(prec)
         for c in cavs:
             e.add(c["lx"], c["ly"], c["lz"], seed=c["seed"], amp=1.0e-3, rayleigh=c["rayleigh"],
-                  prandtl=c["prandtl"], t_hot=0.5, t_cold=-0.5)
+                  prandtl=c["prandtl"], t_hot=0.5, t_cold=-0.5, pattern=args.ensemble_pattern)
         e.step(2)
         t = time.perf_counter()
         if args.tuning_cache:
@@ -177,7 +177,7 @@ def run_ensemble_workload(args) -> dict:
         tot = 0.0
         for c in cavs:
             cv = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], t_hot=0.5,
-                        t_cold=-0.5, precision=prec)
+                        t_cold=-0.5, precision=prec, pattern=args.ensemble_pattern)
             cv.init_conductive(c["seed"], 1.0e-3)
             cv.step(4)
             cv.synchronize()
@@ -404,6 +404,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--config1-steps", type=int, default=1000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
+    ap.add_argument("--ensemble-pattern", default="ab", choices=["ab", "aa"],
+                    help="ensemble members (and the sequential comparison) with two buffers or one in place")
     ap.add_argument("--sequential-steps", type=int, default=300)
     ap.add_argument("--tuning-cache", action="store_true",
                     help="ensemble workload: reuse/record the autotuner's choice in <out-dir>/ensemble_tuning.txt")
@@ -499,7 +501,8 @@ def main() -> int:
             "vs_baseline": None, "dtype": "f64" if main_prec == "fp64" else f"{main_prec} storage, f32 arithmetic",
             "data": "synthetic: 64 cavities, sizes/Ra/Pr drawn with seed 4, cavity k seeded 1000+k",
             "config": {"workload": "64 Rayleigh-Benard cavities, lateral {32..128}, height {32,64,128}, "
-                                   "Ra log-U[1e4,1e7], Pr {0.71,1,4.4}", "cells": res["cells"]},
+                                   "Ra log-U[1e4,1e7], Pr {0.71,1,4.4}", "cells": res["cells"],
+                       "pattern": args.ensemble_pattern},
             "ensemble": res,
         }), flush=True)
         return 0
