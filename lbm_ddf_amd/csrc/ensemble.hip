@@ -305,7 +305,7 @@ lbm_ctx *cavity(lbm_ensemble *e, int32_t slot) {
 }
 
 std::string tuning_key(const lbm_ensemble *e) {
-    // precision + sorted multiset of (lx, ly, lz), hashed
+    // precision, storage pattern + sorted multiset of (lx, ly, lz), hashed
     std::vector<uint64_t> dims;
     for (const lbm_ctx *c : e->slots)
         if (c) dims.push_back((static_cast<uint64_t>(c->p.lx) << 40) | (static_cast<uint64_t>(c->p.ly) << 20) | c->p.lz);
@@ -316,7 +316,7 @@ std::string tuning_key(const lbm_ensemble *e) {
         h *= 1099511628211ull;
     }
     std::ostringstream os;
-    os << "p" << e->prec << "_n" << dims.size() << "_" << std::hex << h;
+    os << "p" << e->prec << (e->aa ? "aa" : "") << "_n" << dims.size() << "_" << std::hex << h;
     return os.str();
 }
 
