@@ -249,7 +249,7 @@ int lbm_ensemble_get_tuning(const lbm_ensemble *ens, int32_t *block_x, int32_t *
  * `steps` steps on the live ensemble, then 2 and 4 cells per thread for the
  * four fastest shapes, re-times the four fastest candidates (twice, over
  * 3*steps steps each) and keeps the fastest of those; with cache_path, a
- * result stored for the same precision and cavity-size multiset is reused
+ * result stored for the same precision, storage pattern and cavity-size multiset is reused
  * without timing. */
 int lbm_ensemble_autotune(lbm_ensemble *ens, int32_t steps, const char *cache_path, double *best_ms_per_step);
 
