@@ -69,7 +69,8 @@ extern "C" {
 #define LBM_PATTERN_AB 0 /* two buffers per distribution, pull streaming */
 #define LBM_PATTERN_AA 1 /* one buffer per distribution, updated in place by
                             alternating even/odd passes: half the population
-                            memory, same results bit for bit */
+                            memory; FP64 results equal A/B bit for bit (16-bit
+                            storage rounds g against other rows: close) */
 
 /* Physical and lattice parameters: the macros of src/parameters.h:8-26
  * made runtime.  beta <= 0 selects the reference default 0.1/(5*LY).
@@ -223,7 +224,8 @@ int lbm_ensemble_destroy(lbm_ensemble *ens);
  * start of lbm_init_conductive; *slot receives its index.  params->pattern
  * must match the resident members' (the first cavity of an empty ensemble
  * sets it): in-place members need half the population memory, so more of
- * them fit a budget, with the same results bit for bit. */
+ * them fit a budget; each member equals its standalone run of the same
+ * pattern bit for bit. */
 int lbm_ensemble_add(lbm_ensemble *ens, const lbm_params *params, uint64_t seed, double amp, int32_t *slot);
 int lbm_ensemble_remove(lbm_ensemble *ens, int32_t slot);
 int lbm_ensemble_count(const lbm_ensemble *ens, int32_t *slots, int32_t *resident, uint64_t *cells);
@@ -305,7 +307,7 @@ typedef struct lbm_serve_options {
                                     (enough cells to saturate HBM, so admitted requests finish) */
     double progress_s;           /* > 0: the loop prints a progress line to stderr at this interval */
     int32_t pattern;             /* LBM_PATTERN_AB (0) or LBM_PATTERN_AA: in place, half the
-                                    population bytes per request, same results */
+                                    population bytes per request (FP64: same results) */
     int32_t reserved;            /* must be 0 */
 } lbm_serve_options;
 

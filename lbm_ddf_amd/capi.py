@@ -502,7 +502,7 @@ def serve(requests: list, precision: str = "fp64", chunk_steps: int = 100, hbm_b
     walls +-0.5, amp 1e-3).  Returns (results list of dicts, stats dict).
     progress_s > 0 makes the loop print a progress line to stderr at that
     interval; pattern "aa" keeps every resident cavity in place (half the
-    population bytes, same results)."""
+    population bytes; FP64 results equal to "ab")."""
     lib = load()
     n = len(requests)
     arr = (Request * max(1, n))()
