@@ -121,3 +121,48 @@ def test_null_context_is_an_error_not_a_crash():
     assert lib.lbm_step(None, 1) == -1
     assert lib.lbm_finalize(None) == -1
     assert lib.lbm_destroy(None) == 0
+
+
+def test_argument_errors_of_the_other_entry_points_need_no_gpu():
+    """Validation happens before any device work: bad arguments come back as
+    LBM_E_INVALID with a message, never as a crash."""
+    lib = capi.load()
+    v = ctypes.c_double()
+    assert lib.lbm_copy_bandwidth(ctypes.c_uint64(100), 5, ctypes.byref(v)) == -1
+    assert lib.lbm_copy_bandwidth(ctypes.c_uint64(1 << 20), 0, ctypes.byref(v)) == -1
+    e = ctypes.c_void_p()
+    assert lib.lbm_ensemble_create(9, ctypes.byref(e)) == -1
+    assert b"precision" in lib.lbm_last_error()
+    assert not e.value
+    assert lib.lbm_ensemble_destroy(None) == 0
+    # the request loop checks its options first
+    reqs = (capi.Request * 1)()
+    reqs[0].lx = reqs[0].ly = reqs[0].lz = 8
+    reqs[0].window = 100
+    reqs[0].max_steps = 100
+    res = (capi.Result * 1)()
+    st = capi.ServeStats()
+
+    def run(**kw):
+        o = dict(precision=0, max_resident=0, chunk_steps=50, hbm_budget=1 << 30, time_limit_s=1.0, block_x=0,
+                 blocks_per_cu=8, flags=7, cells_per_thread=1, max_resident_cells=0, progress_s=0.0, pattern=0,
+                 reserved=0)
+        o.update(kw)
+        opt = capi.ServeOptions(**o)
+        return lib.lbm_serve(reqs, 1, ctypes.byref(opt), res, ctypes.byref(st))
+
+    assert run(chunk_steps=0) == -1 and b"chunk_steps" in lib.lbm_last_error()
+    assert run(chunk_steps=30) == -1 and b"multiple of chunk_steps" in lib.lbm_last_error()
+    assert run(pattern=2) == -1 and b"pattern" in lib.lbm_last_error()
+    assert run(reserved=1) == -1 and b"reserved" in lib.lbm_last_error()
+
+
+def test_cell_count_limit():
+    """Per-cell indices are 32-bit: 2^32 cells or more are refused."""
+    b = ctypes.c_uint64()
+    lib = capi.load()
+    p = capi.make_params(2048, 2048, 1024)  # exactly 2^32
+    assert lib.lbm_bytes_required(ctypes.byref(p), ctypes.byref(b)) == -1
+    assert b"2^32" in lib.lbm_last_error()
+    p = capi.make_params(2048, 2048, 1023)
+    assert lib.lbm_bytes_required(ctypes.byref(p), ctypes.byref(b)) == 0
