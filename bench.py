@@ -545,6 +545,10 @@ def main() -> int:
     if d["nonfinite"] != 0:
         log("WARNING: non-finite values in the final state")
     c.close()
+    # the second roofline reference: a plain device copy (4 GiB, read + write)
+    from lbm_ddf_amd import capi as _capi
+
+    copy_gbps = _capi.copy_bandwidth(4 << 30, 5)
 
     mlups = n * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
@@ -587,6 +591,9 @@ def main() -> int:
             "traffic": traffic,
             "algorithmic_bytes_per_launch": BYTES_PER_CELL[prec] * n,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "copy_gbps": round(copy_gbps, 1),
+            "frac_of_copy": round(achieved / copy_gbps, 4),
+            "traffic_rate_of_copy": round(traffic / avg_launch_s / 1e9 / copy_gbps, 4) if traffic else None,
         },
         "physics": {"nu_bulk": d["nu_bulk"], "nu_top": d["nu_top"], "nu_bottom": d["nu_bottom"],
                     "mass": d["mass"], "kinetic_energy": d["kinetic_energy"],

@@ -372,11 +372,13 @@ hipError_t codec_plane(bool encode, int which, int ip, const void *src, void *ds
     return err;
 }
 
-// Plain device copy, 16 bytes per thread per iteration, grid-stride: the
+// Plain device copy, 16 bytes per thread, one thread per element: the
 // achievable-bandwidth reference and the calibration kernel for HBM counters.
+// (Measured on MI355X over 16 GiB: 6.19 TB/s, against 4.85 TB/s for a
+// grid-stride loop and 5.6-5.9 TB/s for 4 or 8 elements per thread.)
 __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
 }
 
 }  // namespace
@@ -457,8 +459,10 @@ hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s) {
 
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s) {
     const uint64_t n16 = bytes / 16;
-    hipLaunchKernelGGL(copy_kernel, dim3(256 * 32), dim3(256), 0, s, static_cast<const uint4 *>(src),
-                       static_cast<uint4 *>(dst), n16);
+    const uint64_t blocks = (n16 + 255) / 256;
+    if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
     return hipGetLastError();
 }
 
