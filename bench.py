@@ -605,12 +605,12 @@ def main() -> int:
     else:
         out["cpu_baseline"] = None
     if prec == "fp64" and not args.no_fp16:
-        out["fp16_storage"] = fp16_leg(args, mlups)
+        out["fp16_storage"] = fp16_leg(args, mlups, copy_gbps)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def fp16_leg(args, fp64_mlups: float) -> dict:
+def fp16_leg(args, fp64_mlups: float, copy_gbps: float) -> dict:
     """The same cavity with FP16 storage (FP32 arithmetic): MLUPS, roofline
     fraction at 216 B per cell and the speed-up over the FP64 run above."""
     from lbm_ddf_amd import Cavity
@@ -639,7 +639,7 @@ def fp16_leg(args, fp64_mlups: float) -> dict:
     return {"value": round(mlups, 2), "unit": "MLUPS", "steps": steps, "ms_per_step": round(wall / steps * 1e3, 4),
             "speedup_vs_fp64": round(mlups / fp64_mlups, 3),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "frac_of_copy": round(achieved / copy_gbps, 4),
                          "traffic": traffic_from_profiles(f"fused_fp16_{shape}_bx{bx}_f{fl}")
                          or traffic_from_profiles(f"fused_fp16_{shape}"),
                          "algorithmic_bytes_per_launch": BYTES_PER_CELL["fp16"] * n},
