@@ -4,7 +4,9 @@ The oracle cannot follow a 512^3 cavity within a test's time, so at that size
 the FP64 path is checked against itself along routes that must agree bit for
 bit: the in-place pattern against the A/B pattern (different kernels and
 memory layouts), and the fused kernel against the reference's staged
-sequence.  Both contexts together need about 190 GB of the 288 GB HBM.
+sequence.  Both contexts together need about 190 GB of the 288 GB HBM.  The
+full 512x512 cross-section (32 planes deep) is checked against the oracle
+itself.
 """
 import gc
 
@@ -84,4 +86,32 @@ def test_largest_cavity_in_place():
         assert d["max_speed"] < 1e-2
     finally:
         c.close()
+        gc.collect()
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_full_size_xy_planes_against_the_oracle(pattern):
+    """The workload's full 512x512 cross-section (a 512x512x32 cavity: the
+    512^3 launch shape, row-blocked layout and kernel variant) against the
+    FP64 oracle itself after 3 steps, every population and field bit for bit
+    (the oracle's state is about 8 GB of host memory)."""
+    import oracle
+
+    shape = (L, L, 32)
+    c = Cavity(*shape, pattern=pattern, **RB)
+    o = oracle.OracleState(*shape, **RB)
+    try:
+        c.init_conductive(6, 1.0e-2)
+        o.init_conductive(6, 1.0e-2)
+        c.step(3)
+        o.run(3)
+        for k in ("rho", "ux", "uy", "uz", "phi", "fy"):
+            np.testing.assert_array_equal(c.get_field(k), o[k], err_msg=k)
+        for k in ("f", "g"):
+            got = c.get_pops(k)
+            assert np.array_equal(got, o[k]), k
+            del got
+    finally:
+        c.close()
+        del o
         gc.collect()
