@@ -19,9 +19,12 @@
 //     block per cavity sums them in a fixed order);
 //   * a launch-shape autotuner with an on-disk cache.
 //
-// The per-cell arithmetic is step_cell(), the same code as the single-cavity
-// kernel, so a cavity's state after k steps in the ensemble equals a
-// standalone run of k steps bit for bit.
+// The per-cell arithmetic is step_cell() (A/B members) or aa_cell() (in-place
+// members, all members of an ensemble sharing one pattern), the same code as
+// the single-cavity kernels, so a cavity's state after k steps in the
+// ensemble equals a standalone run of k steps bit for bit.  An in-place
+// member runs its even or odd pass in each launch according to its own
+// layout, so members that joined at different times share launches.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
