@@ -339,3 +339,216 @@ def test_perturbation_bounds_and_determinism():
     assert v.std() == pytest.approx(1e-3 / math.sqrt(3), rel=0.02)
     assert oracle.perturbation(1, 0, 1e-3) == v[0]
     assert oracle.perturbation(2, 0, 1e-3) != v[0]
+
+
+# ----------------------------------------------------------------- moments of the collisions
+def _second_moment(pops):
+    """Pi_ab = sum_i c_ia c_ib pops_i, shape (3, 3) + cell shape."""
+    Cf = C.astype(float)
+    return np.einsum("ia,ib,i...->ab...", Cf, Cf, pops)
+
+
+def test_collision_g_first_and_second_moments_closed_form():
+    """g_eq = w phi (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) has sum c_i g_eq = phi u
+    and sum c_a c_b g_eq = phi (delta_ab / 3 + u_a u_b) (lattice isotropy up to
+    4th order), so the BGK change of src/collision.cu:74-81 obeys
+    sum c_i dg_i = -(sum c_i g_i - phi u) / tau_c and the same for the second
+    moment.  A sign slip in c.u, a dropped 1/2 or a wrong 1/RT fails here."""
+    s = _single_cells(seed=5)
+    tauc = s.p.tauc
+    g0 = s["g"].copy()
+    s.collision_BGK_scalar()
+    dg = s["g"] - g0
+    phi = s["phi"]
+    u = np.stack([s["ux"], s["uy"], s["uz"]])
+    Cf = C.astype(float)
+    j0 = np.einsum("ia,i...->a...", Cf, g0)
+    dj = np.einsum("ia,i...->a...", Cf, dg)
+    np.testing.assert_allclose(dj, -(j0 - phi * u) / tauc, rtol=0, atol=1e-15)
+    pi0, dpi = _second_moment(g0), _second_moment(dg)
+    eq = phi * (np.eye(3)[:, :, None, None, None] / 3.0 + u[:, None] * u[None, :])
+    np.testing.assert_allclose(dpi, -(pi0 - eq) / tauc, rtol=0, atol=1e-15)
+
+
+def test_collision_f_second_moment_without_force():
+    """sum c_a c_b f_eq = rho' delta_ab / 3 + u_a u_b (f is stored as a
+    deviation, the 1 of the density sits outside the quadratic terms,
+    src/collision.cu:26): without force the BGK change of the momentum flux is
+    -(Pi - Pi_eq) / tau."""
+    s = _single_cells(seed=6)
+    s["fy"] = 0.0
+    tau = s.p.tau
+    f0 = s["f"].copy()
+    s.collision_BGK()
+    df = s["f"] - f0
+    u = np.stack([s["ux"], s["uy"], s["uz"]])
+    eq = s["rho"] * np.eye(3)[:, :, None, None, None] / 3.0 + u[:, None] * u[None, :]
+    np.testing.assert_allclose(_second_moment(df), -(_second_moment(f0) - eq) / tau, rtol=0, atol=1e-15)
+
+
+# ----------------------------------------------------------------- output_profile
+def test_output_profile_known_planes():
+    """Per-plane means and phi_rms = sqrt(<phi^2> - <phi>^2) of
+    src/output.cu:336-358 on fields built from dyadic values, so that every
+    sum is exact: plane iy holds phi = a_y + b_y s(x, z) with s = +-1 balanced
+    over the plane, giving mean a_y and rms |b_y| exactly; plane 0 is constant
+    (rms exactly 0).  u_x, u_y, u_z are plane constants plus balanced
+    patterns."""
+    LX, LY, LZ = 4, 5, 6
+    s = oracle.OracleState(LX, LY, LZ, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    iz, iy, ix = np.meshgrid(np.arange(LZ), np.arange(LY), np.arange(LX), indexing="ij")
+    sgn = np.where((ix + iz) % 2 == 0, 1.0, -1.0)       # balanced in every plane
+    sgn2 = np.where(ix < LX // 2, 1.0, -1.0)            # balanced too
+    a = 0.25 * iy - 0.5
+    b = 0.125 * iy
+    s["phi"] = a + b * sgn
+    s["ux"] = 0.0625 * iy + 0.5 * sgn2
+    s["uy"] = -0.03125 * iy
+    s["uz"] = 0.75 * sgn
+    prof = s.profile()
+    y = np.arange(LY)
+    np.testing.assert_array_equal(prof[:, 0], 0.0625 * y)
+    np.testing.assert_array_equal(prof[:, 1], -0.03125 * y)
+    np.testing.assert_array_equal(prof[:, 2], np.zeros(LY))
+    np.testing.assert_array_equal(prof[:, 3], 0.25 * y - 0.5)
+    np.testing.assert_array_equal(prof[:, 4], 0.125 * y)
+    assert prof[0, 4] == 0.0
+
+
+def test_output_profile_population_statistics():
+    """On a random field the profile equals numpy's plane mean and population
+    (ddof=0) standard deviation: the rms is about the plane mean, not zero and
+    not with the n-1 normalisation."""
+    LX, LY, LZ = 7, 6, 5
+    s = oracle.OracleState(LX, LY, LZ, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    rng = np.random.default_rng(11)
+    for k in ("ux", "uy", "uz"):
+        s[k] = rng.uniform(-0.1, 0.1, s.shape)
+    s["phi"] = 0.3 + rng.uniform(-0.2, 0.2, s.shape)
+    prof = s.profile()
+    for col, k in enumerate(("ux", "uy", "uz", "phi")):
+        np.testing.assert_allclose(prof[:, col], s[k].mean(axis=(0, 2)), rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(prof[:, 4], s["phi"].std(axis=(0, 2), ddof=0), rtol=1e-9)
+
+
+# ----------------------------------------------------------------- energy and mass
+def test_energy_mass_known_fields():
+    """kinetic energy = 0.5 <|u|^2> over cells, mass = sum rho' (no
+    normalisation): checked on dyadic fields (exact) and against numpy on a
+    random field."""
+    LX, LY, LZ = 4, 3, 2
+    s = oracle.OracleState(LX, LY, LZ, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    s["ux"], s["uy"], s["uz"] = 0.5, -0.25, 0.125
+    s["rho"] = 0.0
+    s["rho"][0, 0, 0] = 0.375
+    s["rho"][1, 2, 3] = -0.125
+    ke, mass = s.energy_mass()
+    assert ke == 0.5 * (0.25 + 0.0625 + 0.015625)
+    assert mass == 0.25
+    rng = np.random.default_rng(12)
+    for k in ("ux", "uy", "uz", "rho"):
+        s[k] = rng.uniform(-0.1, 0.1, s.shape)
+    ke, mass = s.energy_mass()
+    usq = s["ux"] ** 2 + s["uy"] ** 2 + s["uz"] ** 2
+    assert ke == pytest.approx(0.5 * usq.mean(), rel=1e-14)
+    assert mass == pytest.approx(s["rho"].sum(), rel=1e-12, abs=1e-16)
+
+
+# ----------------------------------------------------------------- initial states
+@pytest.mark.parametrize("ly", [5, 8, 33])
+def test_init_conductive_profile(ly):
+    """Conductive start without perturbation: phi falls linearly from T_hot at
+    iy = 0 to T_cold at iy = LY-1 (Parallel/initial.cu:89), the same in every
+    x-z position; rho' = u = 0; F_y = g beta (phi - t0); f = 0 (f_eq of rho' = 0,
+    src/initial.cu:237-238) and g = w phi (:240-241)."""
+    t_hot, t_cold = 0.7, -0.3
+    s = oracle.OracleState(3, ly, 4, rayleigh=1e5, prandtl=0.71, t_hot=t_hot, t_cold=t_cold)
+    s.init_conductive(7, 0.0)
+    phi = s["phi"]
+    assert np.all(phi == phi[0:1, :, 0:1])  # laterally uniform
+    col = phi[0, :, 0]
+    assert col[0] == t_hot and col[-1] == pytest.approx(t_cold, abs=1e-15)
+    d = np.diff(col)
+    np.testing.assert_allclose(d, -(t_hot - t_cold) / (ly - 1), rtol=1e-12)  # constant slope
+    if ly % 2 == 1:
+        assert col[ly // 2] == pytest.approx(0.5 * (t_hot + t_cold), abs=1e-15)
+    for k in ("rho", "ux", "uy", "uz", "fx", "fz"):
+        assert np.all(s[k] == 0.0), k
+    np.testing.assert_array_equal(s["fy"], s.p.grav0 * s.p.beta * (phi - s.p.t0))
+    assert np.all(s["f"] == 0.0)
+    np.testing.assert_array_equal(s["g"], W[:, None, None, None] * phi[None])
+
+
+def test_init_arrays_reference_state():
+    """init_arrays (src/initial.cu:208-242): phi = 0.5 everywhere, rho' = u = 0,
+    f = 0, g = w / 2, F_y = g beta (0.5 - t0)."""
+    s = oracle.OracleState(3, 4, 5, rayleigh=1e5, prandtl=0.71, t_hot=1.0, t_cold=0.0)
+    s.init_arrays()
+    assert np.all(s["phi"] == 0.5)
+    assert np.all(s["f"] == 0.0)
+    np.testing.assert_array_equal(s["g"], np.broadcast_to(W[:, None, None, None] * 0.5, s["g"].shape))
+    assert np.all(s["fy"] == s.p.grav0 * s.p.beta * (0.5 - 0.5))
+
+
+# ----------------------------------------------------------------- perturbation
+_M64 = (1 << 64) - 1
+_GAMMA = 0x9E3779B97F4A7C15
+
+
+class _SplitMix64:
+    """The published SplitMix64 generator (Steele, Lea and Flood, OOPSLA 2014;
+    Vigna's reference splitmix64.c), restated with Python integers: the state
+    advances by the golden gamma, the output is the state passed through the
+    variant-13 mixer."""
+
+    def __init__(self, seed):
+        self.x = seed & _M64
+
+    def next(self):
+        self.x = (self.x + _GAMMA) & _M64
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+
+def test_splitmix64_restatement_matches_published_vectors():
+    """The restatement reproduces the published known answers (Rosetta Code,
+    'Pseudo-random numbers/Splitmix64'): the first five outputs for seed
+    1234567, and the bins of floor(5 u) over 100000 draws for seed 987654321."""
+    r = _SplitMix64(1234567)
+    assert [r.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                            4593380528125082431, 16408922859458223821]
+    r = _SplitMix64(987654321)
+    counts = [0] * 5
+    for _ in range(100000):
+        counts[int(((r.next() >> 11) * 2.0 ** -53) * 5)] += 1
+    assert counts == [20027, 19892, 20073, 19978, 20030]
+
+
+def test_perturbation_is_splitmix64_of_seed_and_cell():
+    """The draw for cell idx is the (idx+1)-th SplitMix64 output of the seed,
+    its top 53 bits as u in [0, 1), returned as amp (2u - 1).  With amp = 1/2
+    every step is exact, so u is recovered bit for bit."""
+    for seed in (1234567, 1, 2 ** 63 + 5):
+        r = _SplitMix64(seed)
+        for idx in range(64):
+            u = (r.next() >> 11) * 2.0 ** -53
+            assert oracle.perturbation(seed, idx, 0.5) + 0.5 == u
+    # the first outputs for seed 1234567 in the form the initial state uses
+    want = [(v >> 11) * 2.0 ** -53 for v in (6457827717110365317, 3203168211198807973, 9817491932198370423)]
+    got = [oracle.perturbation(1234567, i, 1.0e-3) for i in range(3)]
+    np.testing.assert_array_equal(got, [1.0e-3 * (2.0 * u - 1.0) for u in want])
+
+
+def test_init_conductive_perturbation_per_cell():
+    """phi - (conductive profile) is the draw of the cell's own index
+    (idx = (iz LY + iy) LX + ix), not of another cell or seed."""
+    LX, LY, LZ = 3, 4, 2
+    s0 = oracle.OracleState(LX, LY, LZ, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    s0.init_conductive(9, 0.0)
+    s = oracle.OracleState(LX, LY, LZ, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    s.init_conductive(9, 1.0e-3)
+    r = _SplitMix64(9)
+    draws = np.array([1.0e-3 * (2.0 * ((r.next() >> 11) * 2.0 ** -53) - 1.0) for _ in range(LX * LY * LZ)])
+    np.testing.assert_array_equal(s["phi"].ravel(), s0["phi"].ravel() + draws)
