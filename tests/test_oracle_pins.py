@@ -552,3 +552,33 @@ def test_init_conductive_perturbation_per_cell():
     r = _SplitMix64(9)
     draws = np.array([1.0e-3 * (2.0 * ((r.next() >> 11) * 2.0 ** -53) - 1.0) for _ in range(LX * LY * LZ)])
     np.testing.assert_array_equal(s["phi"].ravel(), s0["phi"].ravel() + draws)
+
+
+# ----------------------------------------------------------------- macrovar, forcing
+def test_macrovar_moments_with_half_force():
+    """src/macrovar.cu:3-34: rho' = sum f, u = sum c f + F/2 (F as stored,
+    i.e. from the previous phi), phi = sum g."""
+    s = _single_cells(seed=13)
+    rng = np.random.default_rng(14)
+    s["fx"] = rng.uniform(-1e-3, 1e-3, s.shape)
+    s["fz"] = rng.uniform(-1e-3, 1e-3, s.shape)
+    f, g = s["f"].copy(), s["g"].copy()
+    F = [s["fx"].copy(), s["fy"].copy(), s["fz"].copy()]
+    s.macrovar()
+    np.testing.assert_allclose(s["rho"], f.sum(axis=0), rtol=0, atol=1e-16)
+    np.testing.assert_allclose(s["phi"], g.sum(axis=0), rtol=0, atol=1e-15)
+    for a, k in enumerate(("ux", "uy", "uz")):
+        j = np.einsum("i,i...->...", C[:, a].astype(float), f)
+        np.testing.assert_allclose(s[k], j + 0.5 * F[a], rtol=0, atol=1e-16)
+
+
+def test_forcing_boussinesq():
+    """src/macrovar.cu:36-47: F = (0, g beta (phi - t0), 0)."""
+    s = _single_cells(seed=15)
+    s["fx"] = 1.0
+    s["fz"] = 1.0
+    s.forcing()
+    assert np.all(s["fx"] == 0.0) and np.all(s["fz"] == 0.0)
+    beta = 0.1 / (5 * 4)  # the reference default 0.1 / (5 LY), LY = 4
+    assert s.p.beta == beta
+    np.testing.assert_allclose(s["fy"], 1.0 * beta * (s["phi"] - 0.0), rtol=1e-15)
