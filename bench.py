@@ -121,29 +121,12 @@ def run_config1_workload(args) -> dict:
             "speedup": cpu_wall / gpu_wall, "compare": cmp}
 
 
-def ensemble_cavities(count: int = 64, draw_seed: int = 4):
-    """Config of the 64-cavity ensemble: lateral side from {32,48,64,96,128},
-    height from {32,64,128}, Ra log-uniform in [1e4, 1e7], Pr from
-    {0.71, 1.0, 4.4}, cavity k seeded 1000+k; draws from numpy's PCG64 with
-    seed `draw_seed`."""
-    import numpy as np
-
-    rng = np.random.default_rng(draw_seed)
-    cavs = []
-    for k in range(count):
-        lat = int(rng.choice([32, 48, 64, 96, 128]))
-        h = int(rng.choice([32, 64, 128]))
-        ra = float(10 ** rng.uniform(4.0, 7.0))
-        pr = float(rng.choice([0.71, 1.0, 4.4]))
-        cavs.append(dict(lx=lat, ly=h, lz=lat, rayleigh=ra, prandtl=pr, seed=1000 + k))
-    return cavs
-
-
 def run_ensemble_workload(args) -> dict:
     """64 cavities in one persistent launch per step (hipGraph replay,
     autotuned launch shape) against the same cavities run one after another
     as single-cavity contexts."""
     from lbm_ddf_amd import Cavity, Ensemble
+    from lbm_ddf_amd.workloads import ensemble_cavities
 
     cavs = ensemble_cavities()
     cells = sum(c["lx"] * c["ly"] * c["lz"] for c in cavs)
@@ -216,30 +199,9 @@ def run_ensemble_workload(args) -> dict:
     return out
 
 
-def serve_requests(rate: float, arrivals_s: float, draw_seed: int = 5):
-    """Poisson arrivals at `rate` per second during `arrivals_s` seconds; cube
-    side log-uniform in [32, 256], Ra log-uniform in [1e4, 1e8], Pr from
-    {0.71, 1.0, 7.0}; stop when the running Nu moves < 1e-3 over 1000 steps,
-    or at 50k steps."""
-    import math
-
-    import numpy as np
-
-    rng = np.random.default_rng(draw_seed)
-    reqs, t = [], 0.0
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > arrivals_s:
-            break
-        side = int(round(math.exp(rng.uniform(math.log(32), math.log(256)))))
-        reqs.append(dict(lx=side, ly=side, lz=side, rayleigh=float(10 ** rng.uniform(4.0, 8.0)),
-                         prandtl=float(rng.choice([0.71, 1.0, 7.0])), seed=5000 + len(reqs), amp=1.0e-3,
-                         arrival_s=t, max_steps=50000, nu_tol=1.0e-3, window=1000))
-    return reqs
-
-
 def run_serve_workload(args) -> dict:
     from lbm_ddf_amd import Cavity, serve
+    from lbm_ddf_amd.workloads import serve_requests
 
     reqs = serve_requests(args.serve_rate, args.serve_arrivals)
     cells = [r["lx"] ** 3 for r in reqs]

@@ -216,7 +216,10 @@ int lbm_restart(lbm_ctx *ctx, const char *flow_path, const char *fg_path, int64_
  * steps equals a standalone lbm_step run of k steps bit for bit.  Cavities
  * join (lbm_ensemble_add: conductive start) and leave (lbm_ensemble_remove)
  * between calls.  A member context may be read through the single-cavity
- * calls (lbm_get_field, lbm_reduce, ...) but not stepped on its own. */
+ * calls (lbm_get_field, lbm_reduce, ...) and re-initialised, but not stepped
+ * or destroyed on its own: lbm_step, lbm_step_timed, lbm_run_diag,
+ * lbm_autotune, lbm_step_staged, the stage calls and lbm_destroy return
+ * LBM_E_STATE for a member. */
 typedef struct lbm_ensemble lbm_ensemble;
 int lbm_ensemble_create(int32_t precision, lbm_ensemble **out);
 int lbm_ensemble_destroy(lbm_ensemble *ens);
@@ -317,6 +320,17 @@ typedef struct lbm_serve_stats {
     int32_t completed, rejected, diverged, unfinished, pending, peak_resident;
     uint64_t peak_bytes;
     double p50_s, p99_s; /* time to result (finish - arrival) of completed requests */
+    /* Every request that arrived before the end of the run counts below:
+     * requests still resident or queued at the end with finish = the end of
+     * the run (right-censored: the percentiles are lower bounds of the true
+     * ones), diverged and rejected ones with their retirement time. */
+    int32_t arrived;
+    int32_t deferred;            /* admissions put back because the device allocation
+                                    failed although the budget had room */
+    double p50_all_s, p99_all_s; /* time to result over the arrived requests */
+    /* Means over completed requests: time queued (admit - arrival) and time
+     * resident (finish - admit). */
+    double mean_queue_s, mean_service_s;
 } lbm_serve_stats;
 
 int lbm_serve(const lbm_request *reqs, int32_t n, const lbm_serve_options *opt, lbm_result *results,

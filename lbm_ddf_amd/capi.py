@@ -483,7 +483,9 @@ class ServeStats(ctypes.Structure):
                 ("mlups", ctypes.c_double), ("completed", ctypes.c_int32), ("rejected", ctypes.c_int32),
                 ("diverged", ctypes.c_int32), ("unfinished", ctypes.c_int32), ("pending", ctypes.c_int32),
                 ("peak_resident", ctypes.c_int32), ("peak_bytes", ctypes.c_uint64), ("p50_s", ctypes.c_double),
-                ("p99_s", ctypes.c_double)]
+                ("p99_s", ctypes.c_double), ("arrived", ctypes.c_int32), ("deferred", ctypes.c_int32),
+                ("p50_all_s", ctypes.c_double), ("p99_all_s", ctypes.c_double), ("mean_queue_s", ctypes.c_double),
+                ("mean_service_s", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -53,8 +53,11 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     S *fs = static_cast<S *>(a.fb);
     S *gs = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
-    const T dtref = static_cast<T>(a.dtref);
-    const T tref_y = static_cast<T>(a.tref0 - iy * a.dtref);
+    // reference temperatures of the 16-bit g storage for this row and for
+    // the rows that read what this cell writes (ignored in FP64)
+    const T tref_y = tref_row<T>(a.tref0, a.dtref, iy);
+    const T tref_dst_up = tref_row<T>(a.tref0, a.dtref, dest_row(iy, 1));   // written with c_y = +1
+    const T tref_dst_dn = tref_row<T>(a.tref0, a.dtref, dest_row(iy, -1));  // written with c_y = -1
     // neighbour x - c lies inside for c = +1 (lo) / c = -1 (hi); x + c inside
     // for c = +1 is the hi test and for c = -1 the lo test
     const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -154,7 +157,8 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                     }
                 }
                 // the slot's reader: cell x + c_j, or this cell for a wall value
-                const T tr = inside ? tref_y - static_cast<T>(cy) * dtref : tref_y;
+                const T tref_dst = cy > 0 ? tref_dst_up : (cy < 0 ? tref_dst_dn : tref_y);
+                const T tr = inside ? tref_dst : tref_y;
                 uint64_t dst = at + o * ps;
                 if constexpr (MODE == MODE_AA_ODD) {
                     const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);

@@ -260,6 +260,15 @@ int check_ctx(const lbm_ctx *c) {
     return LBM_OK;
 }
 
+// Calls that advance the state (or free it) are the ensemble's to make for
+// its members.
+int check_own(const lbm_ctx *c, const char *what) {
+    if (int rc = check_ctx(c)) return rc;
+    if (c->member)
+        return fail(LBM_E_STATE, std::string(what) + ": the context is an ensemble member (step it through lbm_ensemble_*)");
+    return LBM_OK;
+}
+
 int require_fp64(const lbm_ctx *c, const char *what) {
     if (c->prec != LBM_PREC_FP64)
         return fail(LBM_E_UNSUPPORTED, std::string(what) + " is available for FP64 contexts only");
@@ -307,6 +316,14 @@ bool valid_flags(bool aa, int flags) {
     return !aa || flags == 0 || flags == 3 || flags == 4 || flags == 7;
 }
 
+// What ctx_create allocates besides the arrays counted in bytes_for: the
+// number of device allocations (two population buffers, six fields, lag, wall
+// rho', partials, partials2, diagnostics records) and the granule each is
+// rounded up to.
+constexpr uint64_t kAllocations = 13;
+constexpr uint64_t kAllocGranule = 4096;
+constexpr uint64_t kDiagRecords = 16;
+
 uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t n = static_cast<uint64_t>(p->lx) * p->ly * p->lz;
     const bool aa = p->pattern == LBM_PATTERN_AA;
@@ -317,8 +334,13 @@ uint64_t bytes_for(const lbm_params *p, int bx) {
     const uint64_t pops = (aa ? 1ull : 2ull) * elems * es;
     const uint64_t fields = 6ull * n * 8ull;
     const uint64_t side = n * side_size(p->precision) + (aa ? 0ull : 2ull * p->lx * p->lz * side_size(p->precision));
+    // diagnostics: per-block partials, their first-level sums, 16 records
+    const uint64_t diag = max_partials_bytes(p) + static_cast<uint64_t>(lbm::kPartialBlocks) * lbm::NQ * 8ull +
+                          kDiagRecords * sizeof(lbm_diag);
+    // each allocation rounded up to the allocator's granule
+    const uint64_t slack = kAllocations * kAllocGranule;
     (void)bx;
-    return pops + fields + side + max_partials_bytes(p);
+    return pops + fields + side + diag + slack;
 }
 
 }  // namespace
@@ -344,6 +366,18 @@ int lbm_bytes_required(const lbm_params *p, uint64_t *bytes) {
 
 int lbm_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
+    if (int rc = check_own(c, "lbm_destroy")) return rc;
+    return lbm::ctx_destroy(c);
+}
+
+int lbm_create(const lbm_params *p, lbm_ctx **out) { return lbm::ctx_create(p, nullptr, false, out); }
+
+}  // extern "C"
+
+namespace lbm {
+
+int ctx_destroy(lbm_ctx *c) {
+    if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipGraphExec_t &g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
@@ -357,12 +391,6 @@ int lbm_destroy(lbm_ctx *c) {
     delete c;
     return LBM_OK;
 }
-
-int lbm_create(const lbm_params *p, lbm_ctx **out) { return lbm::ctx_create(p, nullptr, false, out); }
-
-}  // extern "C"
-
-namespace lbm {
 
 int ctx_fail(int code, const std::string &msg) { return fail(code, msg); }
 
@@ -408,7 +436,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx
     c->bx = default_block(p->lx);
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) {
-        lbm_destroy(c);
+        ctx_destroy(c);
         return hip_fail(e, "hipGetDevice");
     }
     if (stream) {
@@ -420,7 +448,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
-        lbm_destroy(c);
+        ctx_destroy(c);
         return hip_fail(e, "stream/event creation");
     }
     const uint64_t n = c->n;
@@ -447,7 +475,7 @@ int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx
         if (c->aa && (al.ptr == &c->buf[1] || al.ptr == &c->wall_rho)) continue;
         if (hipMalloc(al.ptr, al.bytes) != hipSuccess) {
             (void)hipGetLastError();
-            lbm_destroy(c);
+            ctx_destroy(c);
             return fail(LBM_E_NOMEM, "device allocation of " + std::to_string(al.bytes) + " bytes failed");
         }
     }
@@ -457,9 +485,9 @@ int ctx_create(const lbm_params *p, hipStream_t stream, bool flat_tiles, lbm_ctx
         c->f[k] = c->buf[k];
         c->g[k] = static_cast<char *>(c->buf[k]) + 27ull * c->ps * elem_size(c->prec);
     }
-    rc = ensure_diag_cap(c, 16);
+    rc = ensure_diag_cap(c, kDiagRecords);
     if (rc) {
-        lbm_destroy(c);
+        ctx_destroy(c);
         return rc;
     }
     *out = c;
@@ -493,7 +521,7 @@ int lbm_init_conductive(lbm_ctx *c, uint64_t seed, double amp) {
 // ---- stage interface ------------------------------------------------------
 
 #define LBM_STAGE_PROLOGUE(name)                      \
-    if (int rc = check_ctx(c)) return rc;             \
+    if (int rc = check_own(c, name)) return rc;       \
     if (int rc = require_fp64(c, name)) return rc;    \
     if (int rc = ensure_ref(c)) return rc
 
@@ -572,7 +600,7 @@ int lbm_step_staged(lbm_ctx *c, int64_t nsteps) {
 // ---- fused path -----------------------------------------------------------
 
 int lbm_step(lbm_ctx *c, int64_t nsteps) {
-    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_own(c, "lbm_step")) return rc;
     if (nsteps < 0) return fail(LBM_E_INVALID, "nsteps < 0");
     if (nsteps == 0) return LBM_OK;
     if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
@@ -593,7 +621,7 @@ int lbm_step(lbm_ctx *c, int64_t nsteps) {
 }
 
 int lbm_step_timed(lbm_ctx *c, int64_t nsteps, double *elapsed_ms) {
-    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_own(c, "lbm_step_timed")) return rc;
     if (!elapsed_ms) return fail(LBM_E_INVALID, "elapsed_ms is NULL");
     LBM_HIP(hipEventRecord(c->ev0, c->stream));
     if (int rc = lbm_step(c, nsteps)) return rc;
@@ -606,7 +634,7 @@ int lbm_step_timed(lbm_ctx *c, int64_t nsteps, double *elapsed_ms) {
 }
 
 int lbm_run_diag(lbm_ctx *c, int64_t nsteps, int64_t every, lbm_diag *out, int64_t cap, int64_t *nout) {
-    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_own(c, "lbm_run_diag")) return rc;
     if (nsteps < 0 || every <= 0 || cap < 0 || (cap > 0 && !out))
         return fail(LBM_E_INVALID, "bad nsteps/every/cap/out");
     if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
@@ -711,7 +739,7 @@ int lbm_get_tuning(const lbm_ctx *c, int32_t *block_x, int32_t *flags) {
 }
 
 int lbm_autotune(lbm_ctx *c, int32_t steps, int64_t *steps_taken, double *best_ms) {
-    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_own(c, "lbm_autotune")) return rc;
     if (steps < 1) return fail(LBM_E_INVALID, "steps must be >= 1");
     if (c->phase == PHASE_EMPTY) return fail(LBM_E_STATE, "context has no state: call lbm_init_* first");
     int64_t taken = 0;

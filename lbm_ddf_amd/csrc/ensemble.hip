@@ -351,7 +351,7 @@ int lbm_ensemble_destroy(lbm_ensemble *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
     for (lbm_ctx *c : e->slots)
-        if (c) lbm_destroy(c);
+        if (c) lbm::ctx_destroy(c);
     if (e->cav_dev) (void)hipFree(e->cav_dev);
     if (e->diag_dev) (void)hipFree(e->diag_dev);
     if (e->ctrl_dev) (void)hipFree(e->ctrl_dev);
@@ -376,9 +376,10 @@ int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double
     lbm_ctx *c = nullptr;
     if (int rc = lbm::ctx_create(p, e->stream, true, &c)) return rc;
     if (int rc = lbm_init_conductive(c, seed, amp)) {
-        lbm_destroy(c);
+        lbm::ctx_destroy(c);
         return rc;
     }
+    c->member = true;
     size_t s = 0;
     while (s < e->slots.size() && e->slots[s]) ++s;
     if (s == e->slots.size()) e->slots.push_back(c);
@@ -394,7 +395,7 @@ int lbm_ensemble_remove(lbm_ensemble *e, int32_t slot) {
     lbm_ctx *c = cavity(e, slot);
     if (!c) return ctx_fail(LBM_E_INVALID, "no cavity in that slot");
     (void)hipStreamSynchronize(e->stream);
-    lbm_destroy(c);
+    lbm::ctx_destroy(c);
     e->slots[slot] = nullptr;
     e->tables_dirty = true;
     return LBM_OK;

@@ -202,6 +202,10 @@ struct lbm_ctx {
     int bx = 256;
     int flags = 0;
     int device = 0;
+    // owned by an ensemble: readable through the single-cavity calls, but
+    // stepping or destroying it on its own would desynchronise the
+    // ensemble's descriptor table (the public calls refuse with LBM_E_STATE)
+    bool member = false;
 };
 
 
@@ -212,5 +216,6 @@ StepArgs ctx_args(const lbm_ctx *c, const void *fa, const void *ga, void *fb, vo
 int ctx_prologue(lbm_ctx *c);    // REF -> FUSED (collides the next step)
 int ctx_ensure_ref(lbm_ctx *c);  // FUSED -> REF (finalize)
 int ctx_forcing(lbm_ctx *c);     // forcing kernel on the reference state (any precision)
+int ctx_destroy(lbm_ctx *c);     // frees a context, ensemble member or not
 int ctx_fail(int code, const std::string &msg);
 }  // namespace lbm

@@ -20,7 +20,11 @@
 // backend and a steady clock, and by the CPU tests with a synthetic backend
 // and a virtual clock.  Backend interface:
 //   int need(const lbm_request &q, uint64_t *bytes)   device bytes of a cavity
-//   int add(const lbm_request &q, int32_t *slot)       allocate, conductive start
+//   int add(const lbm_request &q, int32_t *slot)       allocate, conductive start;
+//       LBM_E_NOMEM when the device has less free memory than the budget
+//       assumed: the request goes back to the queue while other cavities are
+//       resident (their retirement frees memory), and is rejected when nothing
+//       is resident (it cannot run at all); the loop keeps serving
 //   void remove(int32_t slot)
 //   int steps(int32_t slot, int64_t *n)                completed steps so far
 //   int chunk(int64_t nsteps, std::vector<lbm_diag> &diag)
@@ -96,7 +100,7 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
     std::vector<lbm_diag> diag;
     std::vector<int64_t> before;
     uint64_t bytes_used = 0, peak_bytes = 0, cells_used = 0;
-    int32_t peak_resident = 0, retired = 0;
+    int32_t peak_resident = 0, retired = 0, deferred = 0;
     double cell_updates = 0.0, busy = 0.0, next_report = opt->progress_s;
     int rc = LBM_OK;
 
@@ -149,7 +153,21 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
             l.req = id;
             l.bytes = need;
             l.cells = cells;
-            if ((rc = be.add(q, &l.slot)) != LBM_OK) break;
+            const int arc = be.add(q, &l.slot);
+            if (arc == LBM_E_NOMEM) {
+                if (live.empty()) {
+                    results[id].status = LBM_REQ_REJECTED;
+                    results[id].finish_s = now;
+                    waiting.erase(waiting.begin() + static_cast<std::ptrdiff_t>(w));
+                    continue;
+                }
+                ++deferred;
+                break;  // admit again after the next chunk's retirements
+            }
+            if (arc != LBM_OK) {
+                rc = arc;
+                break;
+            }
             l.next_check = q.window;
             bytes_used += need;
             cells_used += cells;
@@ -226,7 +244,9 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
 
     if (stats) {
         *stats = lbm_serve_stats{};
-        std::vector<double> ttr;
+        const double wall = clk.now();
+        std::vector<double> ttr, ttr_all;
+        double queue = 0.0, service = 0.0;
         for (int32_t i = 0; i < n; ++i) {
             const lbm_result &r = results[i];
             switch (r.status) {
@@ -234,20 +254,30 @@ int serve_loop(Backend &be, Clock &clk, const lbm_request *reqs, int32_t n, cons
                 case LBM_REQ_BUDGET:
                     stats->completed++;
                     ttr.push_back(r.finish_s - r.arrival_s);
+                    queue += r.admit_s - r.arrival_s;
+                    service += r.finish_s - r.admit_s;
                     break;
                 case LBM_REQ_REJECTED: stats->rejected++; break;
                 case LBM_REQ_DIVERGED: stats->diverged++; break;
                 case LBM_REQ_UNFINISHED: stats->unfinished++; break;
                 default: stats->pending++; break;
             }
+            if (r.arrival_s > wall) continue;  // never arrived
+            stats->arrived++;
+            const bool open = r.status == LBM_REQ_UNFINISHED || r.status == LBM_REQ_PENDING;
+            ttr_all.push_back((open ? wall : r.finish_s) - r.arrival_s);
         }
-        const double wall = clk.now();
         stats->wall_s = wall;
         stats->busy_s = busy;
         stats->cell_updates = cell_updates;
         stats->mlups = wall > 0 ? cell_updates / wall / 1e6 : 0.0;
         stats->p50_s = percentile(ttr, 0.50);
         stats->p99_s = percentile(ttr, 0.99);
+        stats->p50_all_s = percentile(ttr_all, 0.50);
+        stats->p99_all_s = percentile(ttr_all, 0.99);
+        stats->mean_queue_s = stats->completed ? queue / stats->completed : NAN;
+        stats->mean_service_s = stats->completed ? service / stats->completed : NAN;
+        stats->deferred = deferred;
         stats->peak_resident = peak_resident;
         stats->peak_bytes = peak_bytes;
     }

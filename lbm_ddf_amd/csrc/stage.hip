@@ -67,7 +67,7 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
     const double fy = a.fy[idx];
     auto *f = static_cast<typename P::S *>(a.fb);
     auto *g = static_cast<typename P::S *>(a.gb);
-    const typename P::T tref = static_cast<typename P::T>(a.tref0 - iy * a.dtref);
+    const typename P::T tref = tref_row<typename P::T>(a.tref0, a.dtref, iy);
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         const double feq = weight(i) * rho * (1.0 - 1.5 * usq);
@@ -302,10 +302,11 @@ __global__ __launch_bounds__(kProfileThreads) void profile_kernel(StepArgs a, do
     }
 }
 
-// Row reference temperature of the 16-bit g storage at flat cell index i.
+// Row reference temperature of the 16-bit g storage at flat cell index i:
+// the binary32 value the step kernels encode and decode with, in binary64.
 __device__ __forceinline__ double codec_tref(uint64_t i, const CodecArgs &c) {
     const uint64_t iy = (i / static_cast<uint64_t>(c.lx)) % static_cast<uint64_t>(c.ly);
-    return c.tref0 - static_cast<double>(iy) * c.dtref;
+    return static_cast<double>(tref_row<float>(c.tref0, c.dtref, static_cast<int>(iy)));
 }
 
 // Host copies convert in binary64: a decoded value is the stored value

@@ -32,6 +32,7 @@
 
 #include "internal.hpp"
 #include "lattice.hpp"
+#include "tref.hpp"
 
 namespace lbm {
 
@@ -370,9 +371,11 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     S *__restrict__ fb = static_cast<S *>(a.fb);
     S *__restrict__ gb = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
-    // row reference temperature of the 16-bit g storage (ignored in FP64)
-    const T dtref = static_cast<T>(a.dtref);
-    const T tref_y = static_cast<T>(a.tref0 - iy * a.dtref);
+    // reference temperatures of the 16-bit g storage for this row and the
+    // rows below and above (ignored in FP64)
+    const T tref_y = tref_row<T>(a.tref0, a.dtref, iy);
+    const T tref_src_up = tref_row<T>(a.tref0, a.dtref, source_row(iy, 1));   // pulled with c_y = +1
+    const T tref_src_dn = tref_row<T>(a.tref0, a.dtref, source_row(iy, -1));  // pulled with c_y = -1
 
     T f[Q], g[Q];
     T rho, ux, uy, uz, phi;
@@ -431,7 +434,9 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             f[i] = P::dec_f(fr[i]);
             // the value was stored by its source row: iy - c_y when pulled,
             // the cell's own row when bounced back (w_opp(i) = w_i)
-            const T tref = pulled[i] ? tref_y + static_cast<T>(CY[i]) * dtref : tref_y;
+            constexpr int cy = CY[i];
+            const T tref_src = cy > 0 ? tref_src_up : (cy < 0 ? tref_src_dn : tref_y);
+            const T tref = pulled[i] ? tref_src : tref_y;
             g[i] = P::template dec_g<i>(gr[i], tref);
         });
         if (wall) {

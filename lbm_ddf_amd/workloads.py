@@ -1,0 +1,43 @@
+"""Synthetic workloads of the measured configurations, shared by bench.py and
+the GPU tests: the 64-cavity ensemble draw and the Poisson request stream of
+the serving loop.  Draws come from numpy's PCG64 with fixed seeds, so every
+caller gets the same cavities and requests."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def ensemble_cavities(count: int = 64, draw_seed: int = 4) -> list:
+    """Cavities of the ensemble workload: lateral side (x and z) from
+    {32, 48, 64, 96, 128}, height (y) from {32, 64, 128}, Ra log-uniform in
+    [1e4, 1e7], Pr from {0.71, 1.0, 4.4}; cavity k is seeded 1000 + k."""
+    rng = np.random.default_rng(draw_seed)
+    cavs = []
+    for k in range(count):
+        lat = int(rng.choice([32, 48, 64, 96, 128]))
+        h = int(rng.choice([32, 64, 128]))
+        ra = float(10 ** rng.uniform(4.0, 7.0))
+        pr = float(rng.choice([0.71, 1.0, 4.4]))
+        cavs.append(dict(lx=lat, ly=h, lz=lat, rayleigh=ra, prandtl=pr, seed=1000 + k))
+    return cavs
+
+
+def serve_requests(rate: float, arrivals_s: float, draw_seed: int = 5, max_steps: int = 50000,
+                   nu_tol: float = 1.0e-3, window: int = 1000) -> list:
+    """Poisson arrivals at `rate` per second during `arrivals_s` seconds; cube
+    side log-uniform in [32, 256], Ra log-uniform in [1e4, 1e8], Pr from
+    {0.71, 1.0, 7.0}; a request stops when its running Nu moves by less than
+    nu_tol over `window` steps, or at max_steps."""
+    rng = np.random.default_rng(draw_seed)
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > arrivals_s:
+            break
+        side = int(round(math.exp(rng.uniform(math.log(32), math.log(256)))))
+        reqs.append(dict(lx=side, ly=side, lz=side, rayleigh=float(10 ** rng.uniform(4.0, 8.0)),
+                         prandtl=float(rng.choice([0.71, 1.0, 7.0])), seed=5000 + len(reqs), amp=1.0e-3,
+                         arrival_s=t, max_steps=max_steps, nu_tol=nu_tol, window=window))
+    return reqs

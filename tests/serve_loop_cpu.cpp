@@ -30,7 +30,7 @@ struct VirtualClock {
 };
 
 struct FakeBackend {
-    VirtualClock *clk;
+    VirtualClock *clk = nullptr;
     double rate = 1.0e9;  // cell updates per virtual second
     uint64_t bytes_per_cell = 100;
     struct Cav {
@@ -39,6 +39,10 @@ struct FakeBackend {
     };
     std::map<int32_t, Cav> cavs;
     int add_calls = 0;
+    // device memory actually free for cavities (the budget may claim more):
+    // add() fails with LBM_E_NOMEM beyond it
+    uint64_t device_bytes = ~0ull;
+    uint64_t allocated = 0;
 
     static double nu_of(uint64_t seed, int64_t s) {
         if (seed >= 3000) return 4.0;
@@ -51,14 +55,21 @@ struct FakeBackend {
         return 0;
     }
     int add(const lbm_request &q, int32_t *slot) {
+        ++add_calls;
+        uint64_t b = 0;
+        need(q, &b);
+        if (allocated + b > device_bytes) return LBM_E_NOMEM;
         int32_t s = 0;
         while (cavs.count(s)) ++s;
         cavs[s] = Cav{q.seed, static_cast<uint64_t>(q.lx) * q.ly * q.lz};
+        allocated += b;
         *slot = s;
-        ++add_calls;
         return 0;
     }
-    void remove(int32_t slot) { cavs.erase(slot); }
+    void remove(int32_t slot) {
+        allocated -= cavs.at(slot).cells * bytes_per_cell;
+        cavs.erase(slot);
+    }
     int steps(int32_t slot, int64_t *n) {
         *n = cavs.at(slot).steps;
         return 0;
@@ -127,9 +138,11 @@ struct Run {
     int rc = 0;
 };
 
-Run serve(const std::vector<lbm_request> &reqs, const lbm_serve_options &o) {
+Run serve(const std::vector<lbm_request> &reqs, const lbm_serve_options &o, uint64_t device_bytes = ~0ull) {
     VirtualClock clk;
-    FakeBackend be{&clk};
+    FakeBackend be;
+    be.clk = &clk;
+    be.device_bytes = device_bytes;
     Run r;
     r.res.resize(reqs.size());
     r.rc = lbm::serve_loop(be, clk, reqs.data(), static_cast<int32_t>(reqs.size()), &o, r.res.data(), &r.st);
@@ -266,6 +279,62 @@ void test_time_to_result_statistics() {
     CHECK(std::fabs(r.st.cell_updates - cells) < 1e-6 * cells);
 }
 
+void test_allocation_failure_defers_or_rejects() {
+    // The budget admits three 16^3 cavities, the device holds two: the third
+    // admission fails with LBM_E_NOMEM and the request waits for a
+    // retirement instead of ending the run.  A request that the device cannot
+    // hold even alone is rejected, and the others are served.
+    auto o = options();
+    const uint64_t per = 16ull * 16 * 16 * 100;
+    o.hbm_budget = 3 * per;
+    Run r = serve({req(16, 1, 0.0, 500), req(16, 2, 0.0, 900), req(16, 3, 0.0, 500)}, o, 2 * per);
+    CHECK(r.rc == 0);
+    CHECK(r.st.completed == 3 && r.st.rejected == 0);
+    CHECK(r.st.deferred >= 1);
+    CHECK(r.res[2].admit_s >= r.res[0].finish_s);  // after the first retirement freed memory
+    CHECK(r.st.peak_resident == 2);
+    Run alone = serve({req(16, 3, 0.0, 500)}, o);
+    CHECK(same_result(alone.res[0], r.res[2]));
+    Run big = serve({req(20, 1, 0.0, 300), req(10, 2, 0.0, 300)}, o, per);
+    CHECK(big.rc == 0);
+    CHECK(big.res[0].status == LBM_REQ_REJECTED && big.res[1].status == LBM_REQ_BUDGET);
+}
+
+void test_censored_latency_and_queueing() {
+    // Three requests arrive at once; one GPU slot (max_resident 1); the time
+    // limit ends the run while the third is queued.  The completed-only
+    // percentiles see the first request alone; the all-arrived ones count the
+    // resident one and the queued one with the end of the run as their finish.
+    auto o = options();
+    o.max_resident = 1;
+    // 20^3 * 100 / 1e9 = 0.8 ms per chunk; 1001 steps: 8 ms
+    const double chunk = 20.0 * 20 * 20 * 100 / 1.0e9;
+    o.time_limit_s = 14.5 * chunk;
+    Run r = serve({req(20, 1, 0.0, 1000), req(20, 2, 0.0, 100000), req(20, 3, 0.0, 1000),
+                   req(20, 4, 1.0e6, 100)}, o);  // the last never arrives
+    CHECK(r.res[0].status == LBM_REQ_BUDGET);
+    CHECK(r.res[1].status == LBM_REQ_UNFINISHED && r.res[2].status == LBM_REQ_PENDING);
+    CHECK(r.res[3].status == LBM_REQ_PENDING);
+    CHECK(r.st.completed == 1 && r.st.arrived == 3);
+    const double t0 = r.res[0].finish_s - r.res[0].arrival_s;
+    CHECK(r.st.p50_s == t0 && r.st.p99_s == t0);
+    // all arrived: {t0, wall, wall}
+    CHECK(std::fabs(r.st.p50_all_s - r.st.wall_s) < 1e-15);
+    CHECK(r.st.p50_all_s > r.st.p50_s);
+    CHECK(std::fabs(r.st.p99_all_s - r.st.wall_s) < 1e-15);
+    CHECK(r.st.mean_queue_s == 0.0);
+    CHECK(std::fabs(r.st.mean_service_s - t0) < 1e-15);
+    // queueing shows when requests wait: with arrivals at once and one slot,
+    // the second completed request queued for the first one's whole run
+    auto o2 = options();
+    o2.max_resident = 1;
+    Run q = serve({req(10, 1, 0.0, 300), req(10, 2, 0.0, 300)}, o2);
+    CHECK(q.st.completed == 2);
+    const double s0 = q.res[0].finish_s - q.res[0].admit_s;
+    CHECK(std::fabs(q.st.mean_queue_s - 0.5 * (q.res[1].admit_s - q.res[1].arrival_s)) < 1e-15);
+    CHECK(q.res[1].admit_s - q.res[1].arrival_s >= s0);
+}
+
 void test_argument_checks() {
     auto o = options();
     lbm_request q = req(8, 1, 0.0, 100, 1e-3, 150);  // window not a multiple of the chunk
@@ -287,6 +356,8 @@ int main() {
     test_divergence_retires_only_the_sick();
     test_time_limit_and_arrivals();
     test_time_to_result_statistics();
+    test_allocation_failure_defers_or_rejects();
+    test_censored_latency_and_queueing();
     test_argument_checks();
     if (failures) {
         std::printf("%d check(s) failed\n", failures);

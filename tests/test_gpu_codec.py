@@ -2,7 +2,7 @@
 
 FP16 storage keeps f and g - w_i*T_ref(iy) scaled by 2^15 as IEEE binary16;
 BF16 storage keeps the same deviations as binary32 rounded to nearest even
-to 16 bits.  T_ref(iy) = T_hot - dT*(iy + 1/2)/LY.  Host copies convert in
+to 16 bits.  T_ref(iy) = T_hot - dT*(iy + 1/2)/LY, rounded to binary32.  Host copies convert in
 binary64 (a new value rounds through binary32 to the format), so numpy
 reproduces every stored and decoded value exactly: set_pops followed by
 get_pops must return the numpy emulation bit for bit, in both population
@@ -37,7 +37,8 @@ def t_ref(shape):
     lz, ly, lx = shape
     dtref = (WALLS["t_hot"] - WALLS["t_cold"]) / ly
     tref0 = WALLS["t_hot"] - 0.5 * dtref
-    tr = np.array([tref0 - iy * dtref for iy in range(ly)], dtype=np.float64)
+    # the binary32 value every kernel and the host codec use for a row
+    tr = np.array([np.float32(tref0 - iy * dtref) for iy in range(ly)], dtype=np.float64)
     return tr[None, None, :, None]  # broadcast over (27, z, y, x)
 
 

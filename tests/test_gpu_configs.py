@@ -1,0 +1,177 @@
+"""The measured configurations at their own sizes, on the GPU.
+
+* 512^3, Ra=1e8: FP16 and BF16 storage beside the FP64 run of the same seeded
+  start (seed 3).  The oracle cannot follow at this size; the FP64 run is the
+  reference, as in the precision study.  Checked: every moment finite, the
+  temperature field and the Nusselt numbers within the storage rounding of
+  the FP64 run, mass drift within its bound.
+* the 64-cavity ensemble draw (lateral {32..128}, height {32, 64, 128}, Ra
+  log-U[1e4, 1e7], Pr {0.71, 1, 4.4}): members against the oracle bit for
+  bit (FP64, both storage patterns), and against standalone cavities in
+  FP16 in both population layouts.
+* the serving request mix (cube side log-U[32, 256], Ra log-U[1e4, 1e8], Pr
+  {0.71, 1, 7}, Poisson arrivals), with a short step budget: the smallest
+  requests' results against the oracle's diagnostics after the same number
+  of steps, and identical alone and in the mix.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from lbm_ddf_amd import Cavity, Ensemble, serve
+from lbm_ddf_amd.workloads import ensemble_cavities, serve_requests
+
+pytestmark = pytest.mark.gpu
+
+WALLS = dict(t_hot=0.5, t_cold=-0.5)
+
+
+# ---------------------------------------------------------------- 512^3, Ra=1e8, 16-bit storage
+# Bounds after 40 steps from the conductive start.  A stored deviation carries
+# a relative rounding error of 2^-12 (binary16, scaled) or 2^-9 (bfloat16);
+# the deviations of g from w_i T_ref are ~1e-3, those of f ~1e-3 w_i, so each
+# step moves T by at most a few 1e-7 (FP16) or a few 1e-6 (BF16) per cell.
+BOUNDS_512 = {
+    "fp16": dict(max_dT=2e-5, rms_dT=2e-6, mass_per_cell=1e-9, nu_rel=1e-4),
+    "bf16": dict(max_dT=2e-4, rms_dT=2e-5, mass_per_cell=1e-7, nu_rel=1e-3),
+}
+
+
+def test_512_ra1e8_16bit_storage_beside_fp64():
+    n, steps = 512, 40
+    kw = dict(rayleigh=1e8, prandtl=0.71, **WALLS)
+    print(f"config: 512^3 Ra=1e8, FP64 in place + FP16 + BF16, {steps} steps", flush=True)
+    ref = Cavity(n, n, n, pattern="aa", **kw)  # 65.8 GB
+    ref.init_conductive(3, 1.0e-3)
+    ref.step(steps)
+    d64 = ref.reduce()
+    assert d64["nonfinite"] == 0 and d64["step"] == steps
+    phi64 = ref.get_field("phi")
+    ref.close()
+    for prec, b in BOUNDS_512.items():
+        c = Cavity(n, n, n, precision=prec, **kw)
+        c.init_conductive(3, 1.0e-3)
+        c.step(steps)
+        d = c.reduce()
+        dT = np.abs(c.get_field("phi") - phi64)
+        c.close()
+        mass = abs(d["mass"]) / n ** 3
+        rel = {k: abs(d[k] - d64[k]) / abs(d64[k]) for k in ("nu_top", "nu_bottom")}
+        print(f"{prec}: max|dT| {dT.max():.3e} rms {np.sqrt(np.mean(dT ** 2)):.3e} mass/cell {mass:.3e} "
+              f"Nu_top rel {rel['nu_top']:.3e} Nu_bottom rel {rel['nu_bottom']:.3e} "
+              f"(FP64 mass/cell {abs(d64['mass']) / n ** 3:.3e})", flush=True)
+        assert d["nonfinite"] == 0 and d["step"] == steps
+        assert dT.max() < b["max_dT"], prec
+        assert np.sqrt(np.mean(dT ** 2)) < b["rms_dT"], prec
+        assert mass < b["mass_per_cell"], prec
+        for k, v in rel.items():
+            assert v < b["nu_rel"], (prec, k)
+
+
+# ---------------------------------------------------------------- the 64-cavity ensemble draw
+def _ensemble(prec, pattern, cavs):
+    e = Ensemble(prec)
+    slots = [e.add(c["lx"], c["ly"], c["lz"], seed=c["seed"], amp=1.0e-3, rayleigh=c["rayleigh"],
+                   prandtl=c["prandtl"], pattern=pattern, **WALLS) for c in cavs]
+    return e, slots
+
+
+def _picks(cavs):
+    """Indices of the smallest and largest cavity and of one cavity of every
+    lateral size (both population layouts in 16-bit storage)."""
+    cells = [c["lx"] * c["ly"] * c["lz"] for c in cavs]
+    picks = {int(np.argmin(cells)), int(np.argmax(cells))}
+    for lat in (32, 48, 64, 96, 128):
+        picks.add(next(k for k, c in enumerate(cavs) if c["lx"] == lat))
+    return sorted(picks)
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_64_cavity_draw_members_equal_oracle(pattern):
+    cavs = ensemble_cavities()
+    assert len(cavs) == 64
+    print(f"config: 64-cavity draw, FP64, pattern {pattern}, prologue + 10 launches", flush=True)
+    e, slots = _ensemble("fp64", pattern, cavs)
+    e.set_tuning(128, 8, 4, 7, 1)
+    e.step(10)
+    diags = e.run_diag(1)
+    assert all(diags[s]["nonfinite"] == 0 for s in slots)
+    for k in _picks(cavs):
+        c = cavs[k]
+        m = e.cavity(slots[k])
+        # the reducing launch completed step 11 (its diagnostics) and
+        # collided step 12, which the read below completes
+        d = diags[slots[k]]
+        assert d["step"] == 11 and m.step_count == 12
+        o = oracle.OracleState(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], **WALLS)
+        o.init_conductive(c["seed"], 1.0e-3)
+        o.run(11)
+        od = o.diag()
+        for key in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy", "mass"):
+            assert np.isclose(d[key], od[key], rtol=1e-11, atol=1e-14), (k, key, d[key], od[key])
+        o.run(1)
+        s = m.state()
+        for key in ("f", "g", "rho", "ux", "uy", "uz", "phi"):
+            assert np.array_equal(s[key], o[key]), (k, (c["lx"], c["ly"], c["lz"]), key)
+    e.close()
+
+
+def test_64_cavity_draw_fp16_members_equal_standalone():
+    cavs = ensemble_cavities()
+    print("config: 64-cavity draw, FP16 storage, prologue + 10 launches", flush=True)
+    e, slots = _ensemble("fp16", "ab", cavs)
+    e.set_tuning(256, 16, 4, 3, 4)
+    e.step(10)
+    for k in _picks(cavs):
+        c = cavs[k]
+        m = e.cavity(slots[k])
+        ref = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], precision="fp16",
+                     **WALLS)
+        ref.init_conductive(c["seed"], 1.0e-3)
+        ref.step(m.step_count)
+        a, b = m.state(), ref.state()
+        ref.close()
+        for key in a:
+            assert np.array_equal(a[key], b[key]), (k, (c["lx"], c["ly"], c["lz"]), key)
+    e.close()
+
+
+# ---------------------------------------------------------------- the serving request mix
+def _serve_reqs():
+    # about a dozen requests of the serving mix arriving within 0.6 s, each
+    # with a 300-step budget checked every 100 steps
+    return serve_requests(20.0, 0.6, max_steps=300, window=100)
+
+
+def test_serving_mix_results_equal_oracle_and_alone():
+    reqs = _serve_reqs()
+    assert len(reqs) >= 8
+    print(f"config: serving mix, {len(reqs)} requests, sides {[r['lx'] for r in reqs]}, FP64", flush=True)
+    res, st = serve(reqs, precision="fp64", chunk_steps=50, time_limit_s=300.0)
+    assert st["pending"] == 0 and st["unfinished"] == 0 and st["rejected"] == 0
+    assert st["arrived"] == len(reqs)
+    for r in res:
+        assert r["status"] in ("budget", "converged", "diverged")
+    small = sorted(range(len(reqs)), key=lambda k: reqs[k]["lx"])[:2]
+    for k in small:
+        q, r = reqs[k], res[k]
+        assert r["status"] != "diverged"
+        o = oracle.OracleState(q["lx"], q["ly"], q["lz"], rayleigh=q["rayleigh"], prandtl=q["prandtl"], **WALLS)
+        o.init_conductive(q["seed"], q["amp"])
+        o.run(r["steps"])
+        od = o.diag()
+        for key in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+            assert np.isclose(r[key], od[key], rtol=1e-11, atol=1e-14), (k, key, r[key], od[key])
+        alone, _ = serve([dict(q, arrival_s=0.0)], precision="fp64", chunk_steps=50)
+        for key in ("status", "steps", "nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+            assert alone[0][key] == r[key], (k, key)
+
+
+def test_serving_mix_fp16_alone_equals_in_mix():
+    reqs = _serve_reqs()
+    res, st = serve(reqs, precision="fp16", chunk_steps=50, time_limit_s=300.0)
+    assert st["pending"] == 0 and st["unfinished"] == 0
+    for k in sorted(range(len(reqs)), key=lambda k: reqs[k]["lx"])[:2]:
+        alone, _ = serve([dict(reqs[k], arrival_s=0.0)], precision="fp16", chunk_steps=50)
+        for key in ("status", "steps", "nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+            assert alone[0][key] == res[k][key], (k, key)

@@ -227,3 +227,44 @@ def test_members_share_one_storage_pattern():
     e.step(6)
     assert assert_member_equals(e, s, c, pattern="aa") == 7
     e.close()
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_members_equal_oracle(pattern):
+    """Every member of the odd-shaped set (population planes and row-blocked
+    lines, x-lines of 7 to 256 cells) against the FP64 oracle directly."""
+    import oracle
+
+    e, slots = build(pattern=pattern)
+    e.set_tuning(64, 4, 2, 7, 2)
+    e.step(17)
+    for s, cav in zip(slots, CAVS):
+        m = e.cavity(s)
+        assert m.step_count == 18  # prologue + 17 launches
+        o = oracle.OracleState(*cav["shape"], rayleigh=cav["rayleigh"], prandtl=cav["prandtl"], **WALLS)
+        o.init_conductive(cav["seed"], AMP)
+        o.run(18)
+        st = m.state()
+        for key in ("f", "g", "rho", "ux", "uy", "uz", "phi"):
+            assert np.array_equal(st[key], o[key]), (cav["shape"], pattern, key)
+    e.close()
+
+
+def test_member_cannot_be_stepped_or_destroyed_alone():
+    """A member's buffers and parity belong to the ensemble's descriptor
+    table: the calls that advance or free a context refuse a member, and the
+    ensemble stays exact afterwards."""
+    from lbm_ddf_amd import LbmError
+
+    e, slots = build(cavs=CAVS[:2])
+    e.step(4)
+    m = e.cavity(slots[0])
+    for call in (lambda: m.step(1), lambda: m.step_timed(1), lambda: m.run_diag(2, 1), lambda: m.autotune(1),
+                 lambda: m.step_staged(1), lambda: m.forcing(), lambda: m.streaming()):
+        with pytest.raises(LbmError, match="ensemble member"):
+            call()
+    assert m.lib.lbm_destroy(m.h) == -4  # LBM_E_STATE
+    e.step(3)
+    for s, cav in zip(slots, CAVS[:2]):
+        assert assert_member_equals(e, s, cav) == 9  # (prologue + 4) + (prologue + 3)
+    e.close()

@@ -133,3 +133,21 @@ def test_in_place_serving_admits_more_under_one_budget():
     assert st_aa["completed"] == st_ab["completed"] == 6
     for r1, r2 in zip(res_ab, res_aa):
         same(r1, r2)
+
+
+def test_request_results_equal_oracle():
+    """A served request's diagnostics at its last check equal the oracle's
+    after the same number of steps (FP64; the reductions sum in another
+    order, hence 1e-11)."""
+    import oracle
+
+    res, _ = serve([B, dict(A, arrival_s=0.02), D], chunk_steps=50)
+    for q, r in ((A, res[1]), (D, res[2])):
+        assert r["status"] == "budget"
+        o = oracle.OracleState(q["lx"], q["ly"], q["lz"], rayleigh=q["rayleigh"], prandtl=0.71, t_hot=0.5,
+                               t_cold=-0.5)
+        o.init_conductive(q["seed"], q["amp"])
+        o.run(r["steps"])
+        od = o.diag()
+        for k in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+            assert np.isclose(r[k], od[k], rtol=1e-11, atol=1e-14), (q["seed"], k, r[k], od[k])
