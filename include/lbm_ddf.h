@@ -171,6 +171,25 @@ int lbm_run_diag(lbm_ctx *ctx, int64_t nsteps, int64_t every, lbm_diag *out, int
 int lbm_finalize(lbm_ctx *ctx);
 /* Diagnostics of the current state (output_nu, diag_flow; src/output.cu). */
 int lbm_reduce(lbm_ctx *ctx, lbm_diag *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops that must not touch the
+ * allocator, harnesses that forbid it): the diagnostics of lbm_reduce over
+ * moment fields the caller holds in device memory (rho', u_x, u_y, u_z, phi:
+ * LX*LY*LZ doubles each, x fastest), written as one lbm_diag to `out` in
+ * device memory with out->step = step, asynchronously on `stream` (a
+ * hipStream_t; NULL: the null stream).  Only p's geometry, walls, Ra, Pr,
+ * gravity and beta are used.  Nothing is allocated: the partial sums live in
+ * a fixed device array of the library (8.1 MiB on each GPU it runs on, held
+ * until the process ends), which bounds the partial records
+ * ceil(LX/B)*LY*LZ, B = 64, 128 or 256 for LX <= 64, <= 128 or larger, to
+ * LBM_NOALLOC_MAX_RECORDS (LY*LZ <= 65536 for LX <= 256); larger shapes
+ * return LBM_E_INVALID.  The record equals lbm_reduce's for the same fields
+ * bit for bit.  Calls that overlap on one GPU share the array and race; the
+ * first call on a device resolves the array's address (make it before any
+ * capture). */
+#define LBM_NOALLOC_MAX_RECORDS 65536
+int lbm_reduce_fields_noalloc(const lbm_params *p, const double *rho, const double *ux, const double *uy,
+                              const double *uz, const double *phi, int64_t step, lbm_diag *out, void *stream);
 /* Per-y-plane <ux>,<uy>,<uz>,<phi>,phi_rms (output_profile,
  * src/output.cu:282-377); out holds 5*LY doubles. */
 int lbm_profile(lbm_ctx *ctx, double *out);

@@ -120,6 +120,9 @@ SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                                ctypes.POINTER(ctypes.c_int32)]),
     "lbm_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, _D]),
+    # device pointers as integers (c_void_p): fields, the output record, the stream
+    "lbm_reduce_fields_noalloc": (ctypes.c_int, [ctypes.POINTER(Params)] + [ctypes.c_void_p] * 5 +
+                                  [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
