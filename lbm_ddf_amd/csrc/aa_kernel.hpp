@@ -41,6 +41,7 @@ template <class P, int MODE, bool REDUCE, int FLAGS>
 __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
+    constexpr bool EX = P::exact;
     using S = typename P::S;
     using T = typename P::T;
     using L = typename P::L;
@@ -53,11 +54,12 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     S *fs = static_cast<S *>(a.fb);
     S *gs = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
-    // reference temperatures of the 16-bit g storage for this row and for
-    // the rows that read what this cell writes (ignored in FP64)
+    // 16-bit g storage: this row's reference temperature and the offsets of
+    // the rows that read what this cell writes (ignored in FP64).  In
+    // registers g is carried as deviations from w_i*T_c (see step_cell).
     const T tref_y = tref_row<T>(a.tref0, a.dtref, iy);
-    const T tref_dst_up = tref_row<T>(a.tref0, a.dtref, dest_row(iy, 1));   // written with c_y = +1
-    const T tref_dst_dn = tref_row<T>(a.tref0, a.dtref, dest_row(iy, -1));  // written with c_y = -1
+    const T dref_up = tref_row<T>(a.tref0, a.dtref, dest_row(iy, 1)) - tref_y;   // written with c_y = +1
+    const T dref_dn = tref_row<T>(a.tref0, a.dtref, dest_row(iy, -1)) - tref_y;  // written with c_y = -1
     // neighbour x - c lies inside for c = +1 (lo) / c = -1 (hi); x + c inside
     // for c = +1 is the hi test and for c = -1 the lo test
     const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -66,7 +68,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     L *lag = static_cast<L *>(a.lag);
 
     T f[Q], g[Q];
-    T rho, ux, uy, uz, phi;
+    T rho, ux, uy, uz, phi, dsum = T(0);
     {
         S fr[Q], gr[Q];
         static_for<Q>([&](auto I) {
@@ -88,7 +90,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             f[i] = P::dec_f(fr[i]);
-            g[i] = P::template dec_g<i>(gr[i], tref_y);
+            g[i] = EX ? P::template dec_g<i>(gr[i], tref_y) : P::dec_f(gr[i]);
         });
     }
 
@@ -98,11 +100,11 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
         uy = static_cast<T>(a.uy[idx]);
         uz = static_cast<T>(a.uz[idx]);
         phi = static_cast<T>(a.phi[idx]);
+        if constexpr (!EX) dsum = static_cast<T>(a.phi[idx] - static_cast<double>(tref_y));
     } else {
         // moments, summed in direction order (src/macrovar.cu:17-27)
-        T jx = T(0), jy = T(0), jz = T(0);
+        T jx = T(0), jy = T(0), jz = T(0), sg = T(0);
         rho = T(0);
-        phi = T(0);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             rho = rho + f[i];
@@ -112,8 +114,14 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             if constexpr (CY[i] < 0) jy = jy - f[i];
             if constexpr (CZ[i] > 0) jz = jz + f[i];
             if constexpr (CZ[i] < 0) jz = jz - f[i];
-            phi = phi + g[i];
+            sg = sg + g[i];
         });
+        if constexpr (EX) {
+            phi = sg;
+        } else {
+            dsum = sg;
+            phi = tref_y + dsum;
+        }
         const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
         ux = jx;
         uy = jy + T(0.5) * fprev;
@@ -123,24 +131,26 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             a.ux[idx] = static_cast<double>(ux);
             a.uy[idx] = static_cast<double>(uy);
             a.uz[idx] = static_cast<double>(uz);
-            a.phi[idx] = static_cast<double>(phi);
+            a.phi[idx] = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
             a.fy[idx] = static_cast<double>(fprev);
         }
     }
 
     if constexpr (MODE != MODE_AA_MOMENTS) {
         // forcing (src/macrovar.cu:45) and collision
-        const T dphi = phi - t0;
+        const T dphi = EX ? phi - t0 : (tref_y - t0) + dsum;
         const T F = static_cast<T>(a.grav_beta) * dphi;
-        collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
+        collide_cell<P>(f, g, rho, ux, uy, uz, phi, dsum, F, a);
         const T rw = rho;  // rho' of this step: read by the isothermal walls of the next streaming
-        const T tw = static_cast<T>(iy == 0 ? a.t_hot : a.t_cold);
+        const double tw_d = iy == 0 ? a.t_hot : a.t_cold;
+        const T tw = static_cast<T>(tw_d);
+        const T twc = EX ? tw : static_cast<T>(tw_d - static_cast<double>(tref_y));  // T_w - T_c
         static_for<Q>([&](auto I) {
             constexpr int j = I;
             constexpr int cx = CX[j], cy = CY[j], cz = CZ[j], o = OPP[j];
             if constexpr (j == 0) {
                 st<NTS>(&fs[at], P::enc_f(f[0]));
-                st<NTS>(&gs[at], P::template enc_g<0>(g[0], tref_y));
+                st<NTS>(&gs[at], EX ? P::template enc_g<0>(g[0], tref_y) : P::enc_f(g[0]));
             } else {
                 const bool dx = cx == 0 ? true : (cx > 0 ? xhi : xlo);
                 const bool dy = cy == 0 ? true : (cy > 0 ? yhi : ylo);
@@ -153,27 +163,32 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                     // also beyond a z wall (z walls are applied last).
                     if (!dy && dz) {
                         constexpr T w2 = static_cast<T>(2.0 * weight(j));
-                        gv = -gv + w2 * tw * (T(1) + rw);
+                        if constexpr (EX) gv = -gv + w2 * tw * (T(1) + rw);
+                        else gv = -gv + w2 * (twc + tw * rw);
                     }
                 }
-                // the slot's reader: cell x + c_j, or this cell for a wall value
-                const T tref_dst = cy > 0 ? tref_dst_up : (cy < 0 ? tref_dst_dn : tref_y);
-                const T tr = inside ? tref_dst : tref_y;
+                // the slot's reader: cell x + c_j, or this cell for a wall
+                // value; 16-bit g is stored against the reader's row
+                if constexpr (!EX && cy != 0) {
+                    constexpr T w = static_cast<T>(weight(j));
+                    if (inside) gv = gv - w * (cy > 0 ? dref_up : dref_dn);
+                }
                 uint64_t dst = at + o * ps;
                 if constexpr (MODE == MODE_AA_ODD) {
                     const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
                     if (inside) dst = at + j * ps + static_cast<uint64_t>(d);
                 }
                 st<NTS>(&fs[dst], P::enc_f(f[j]));
-                st<NTS>(&gs[dst], P::template enc_g<j>(gv, tr));
+                st<NTS>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
             }
         });
         lag[idx] = P::enc_side(dphi);
     }
 
     if constexpr (REDUCE && MODE != MODE_AA_PROLOGUE) {
+        const double phid = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
         diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux), static_cast<double>(uy),
-                   static_cast<double>(uz), static_cast<double>(phi));
+                   static_cast<double>(uz), phid);
     }
 }
 

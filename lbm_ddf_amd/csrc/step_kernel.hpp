@@ -240,13 +240,26 @@ __device__ __forceinline__ void diag_finish_block(const double *partials, int64_
 
 // ---------------------------------------------------------------------------
 // Collision of one cell, f and g (src/collision.cu:22-32 and :74-81).
+//
+// FP64: the reference's operation order on full values.  16-bit storage:
+// FP32 arithmetic on deviations.  f is a deviation already (the reference
+// stores f with rho0 = 1 taken out).  g arrives as g_i - w_i*T_c with T_c the
+// cell's row reference temperature (tref_row), and phi is passed split as
+// T_c + dsum, dsum = sum of the g deviations.  The collision is then
+//   g_i - w_i T_c  <-  d_i - (d_i - w_i (dsum + phi E_i)) / tau_c,
+//   E_i = e_i + (e_i^2 - |u|^2/c_s^2) / 2,
+// which is the reference's g_i - (g_i - w_i phi (1 + E_i)) / tau_c minus
+// w_i T_c: every FP32 operation acts on quantities of the size of the
+// deviations, so no rounding of a number of size |T| ~ 0.5 enters the
+// stored values (rounding w_i T_c, a constant per row and direction, would
+// add a heat source coherent along each row).
 // ---------------------------------------------------------------------------
 
 template <class P>
 __device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::T (&g)[Q],
                                              typename P::T rho, typename P::T ux, typename P::T uy,
-                                             typename P::T uz, typename P::T phi, typename P::T F,
-                                             const StepArgs &a) {
+                                             typename P::T uz, typename P::T phi, typename P::T dsum,
+                                             typename P::T F, const StepArgs &a) {
     using T = typename P::T;
     if constexpr (P::exact) {
         // Reference operation order, FP64.
@@ -280,8 +293,9 @@ __device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::
             const double geq = (w * phi) * ((1.0 + e) + A);
             g[i] = g[i] + (-(g[i] - geq) / tauc);
         });
+        (void)dsum;
     } else {
-        // FP32 arithmetic with reciprocals; same model.
+        // FP32 arithmetic with reciprocals; same model, g as deviations.
         const float omega = static_cast<float>(1.0 / a.tau);
         const float omegac = static_cast<float>(1.0 / a.tauc);
         const float uv = (ux * ux + uy * uy + uz * uz) * 3.0f;
@@ -306,8 +320,8 @@ __device__ __forceinline__ void collide_cell(typename P::T (&f)[Q], typename P::
             const float feq = w * ((rho + e) + A);
             const float feq0 = w * ((1.0f + e) + A);
             f[i] = f[i] - (f[i] - feq) * omega + pf[CY[i] + 1] * feq0;
-            const float geq = (w * phi) * ((1.0f + e) + A);
-            g[i] = g[i] - (g[i] - geq) * omegac;
+            const float geq_dev = w * (dsum + phi * (e + A));  // g_eq - w T_c
+            g[i] = g[i] - (g[i] - geq_dev) * omegac;
         });
     }
     (void)sizeof(T);
@@ -354,10 +368,17 @@ __device__ __forceinline__ void st(S *p, S v) {
 // One lattice cell of a step pass: gather (or plain load in the prologue),
 // moments, forcing, collision, stores, side values, and the cell's terms of
 // the diagnostics.  Shared by the single-cavity and the ensemble kernels.
+//
+// With 16-bit storage g is carried as deviations from w_i*T_c, T_c the row
+// reference temperature of this cell (see collide_cell): a value pulled
+// from the row below or above, stored against that row's temperature, is
+// moved to this row's by adding w_i*(T_src - T_c), a binary32 difference of
+// two row temperatures that is exact; phi is T_c + (sum of the deviations).
 template <class P, int MODE, bool REDUCE, int FLAGS>
 __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
+    constexpr bool EX = P::exact;
     using S = typename P::S;
     using T = typename P::T;
     using L = typename P::L;
@@ -371,14 +392,15 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     S *__restrict__ fb = static_cast<S *>(a.fb);
     S *__restrict__ gb = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
-    // reference temperatures of the 16-bit g storage for this row and the
-    // rows below and above (ignored in FP64)
+    // 16-bit g storage: this row's reference temperature and the offsets of
+    // the source rows' (ignored in FP64)
     const T tref_y = tref_row<T>(a.tref0, a.dtref, iy);
-    const T tref_src_up = tref_row<T>(a.tref0, a.dtref, source_row(iy, 1));   // pulled with c_y = +1
-    const T tref_src_dn = tref_row<T>(a.tref0, a.dtref, source_row(iy, -1));  // pulled with c_y = -1
+    const T dref_up = tref_row<T>(a.tref0, a.dtref, source_row(iy, 1)) - tref_y;   // pulled with c_y = +1
+    const T dref_dn = tref_row<T>(a.tref0, a.dtref, source_row(iy, -1)) - tref_y;  // pulled with c_y = -1
 
     T f[Q], g[Q];
-    T rho, ux, uy, uz, phi;
+    // rho', u, phi; dsum = phi - T_c with 16-bit storage
+    T rho, ux, uy, uz, phi, dsum = T(0);
     const bool wall_lo = iy == 0, wall_hi = iy == ly - 1;
     const uint64_t wall_idx = (static_cast<uint64_t>(wall_hi ? 1 : 0) * lz + iz) * lx + ix;
     L *__restrict__ wall_rho = static_cast<L *>(a.wall_rho);
@@ -388,13 +410,14 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             f[i] = P::dec_f(ld<NTL>(&fa[at + i * ps]));
-            g[i] = P::template dec_g<i>(ld<NTL>(&ga[at + i * ps]), tref_y);
+            g[i] = EX ? P::template dec_g<i>(ld<NTL>(&ga[at + i * ps]), tref_y) : P::dec_f(ld<NTL>(&ga[at + i * ps]));
         });
         rho = static_cast<T>(a.rho[idx]);
         ux = static_cast<T>(a.ux[idx]);
         uy = static_cast<T>(a.uy[idx]);
         uz = static_cast<T>(a.uz[idx]);
         phi = static_cast<T>(a.phi[idx]);
+        if constexpr (!EX) dsum = static_cast<T>(a.phi[idx] - static_cast<double>(tref_y));
     } else {
         // ---- gather (pull) with the wall rules -------------------------
         const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -431,20 +454,30 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         });
         static_for<Q>([&](auto I) {
             constexpr int i = I;
+            constexpr int cy = CY[i];
             f[i] = P::dec_f(fr[i]);
             // the value was stored by its source row: iy - c_y when pulled,
             // the cell's own row when bounced back (w_opp(i) = w_i)
-            constexpr int cy = CY[i];
-            const T tref_src = cy > 0 ? tref_src_up : (cy < 0 ? tref_src_dn : tref_y);
-            const T tref = pulled[i] ? tref_src : tref_y;
-            g[i] = P::template dec_g<i>(gr[i], tref);
+            if constexpr (EX) {
+                g[i] = P::template dec_g<i>(gr[i], tref_y);
+            } else {
+                g[i] = P::dec_f(gr[i]);
+                if constexpr (cy != 0) {
+                    constexpr T w = static_cast<T>(weight(i));
+                    if (pulled[i]) g[i] = g[i] + w * (cy > 0 ? dref_up : dref_dn);
+                }
+            }
         });
         if (wall) {
-            // Isothermal anti-bounce-back at the y walls for g.  The reference
-            // applies the z walls after the y walls, so a direction whose
-            // source is also outside in z keeps the adiabatic bounce-back.
+            // Isothermal anti-bounce-back at the y walls for g,
+            // -g_opp + 2 w_i T_w (1 + rho').  The reference applies the z walls
+            // after the y walls, so a direction whose source is also outside in
+            // z keeps the adiabatic bounce-back.  As deviations from w_i T_c:
+            // -d_opp + 2 w_i ((T_w - T_c) + T_w rho').
             const T rw = P::dec_side(rw_raw);
-            const T tw = static_cast<T>(wall_lo ? a.t_hot : a.t_cold);
+            const double tw_d = wall_lo ? a.t_hot : a.t_cold;
+            const T tw = static_cast<T>(tw_d);
+            const T twc = EX ? tw : static_cast<T>(tw_d - static_cast<double>(tref_y));
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 constexpr int cy = CY[i], cz = CZ[i];
@@ -453,15 +486,15 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
                     const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
                     if (!iny && inz) {
                         constexpr T w2 = static_cast<T>(2.0 * weight(i));
-                        g[i] = -g[i] + w2 * tw * (T(1) + rw);
+                        if constexpr (EX) g[i] = -g[i] + w2 * tw * (T(1) + rw);
+                        else g[i] = -g[i] + w2 * (twc + tw * rw);
                     }
                 }
             });
         }
         // ---- moments, summed in direction order (src/macrovar.cu:17-27)
-        T jx = T(0), jy = T(0), jz = T(0);
+        T jx = T(0), jy = T(0), jz = T(0), sg = T(0);
         rho = T(0);
-        phi = T(0);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             rho = rho + f[i];
@@ -471,8 +504,14 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             if constexpr (CY[i] < 0) jy = jy - f[i];
             if constexpr (CZ[i] > 0) jz = jz + f[i];
             if constexpr (CZ[i] < 0) jz = jz - f[i];
-            phi = phi + g[i];
+            sg = sg + g[i];
         });
+        if constexpr (EX) {
+            phi = sg;
+        } else {
+            dsum = sg;
+            phi = tref_y + dsum;
+        }
         const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
         ux = jx;
         uy = jy + T(0.5) * fprev;
@@ -481,34 +520,35 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
-                st<NTS>(&gb[at + i * ps], P::template enc_g<i>(g[i], tref_y));
+                st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
             });
             a.rho[idx] = static_cast<double>(rho);
             a.ux[idx] = static_cast<double>(ux);
             a.uy[idx] = static_cast<double>(uy);
             a.uz[idx] = static_cast<double>(uz);
-            a.phi[idx] = static_cast<double>(phi);
+            a.phi[idx] = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
             a.fy[idx] = static_cast<double>(fprev);
         }
     }
 
     if constexpr (MODE != MODE_FINALIZE) {
         // ---- forcing (src/macrovar.cu:45) and collision -----------------
-        const T dphi = phi - t0;
+        const T dphi = EX ? phi - t0 : (tref_y - t0) + dsum;
         const T F = static_cast<T>(a.grav_beta) * dphi;
-        collide_cell<P>(f, g, rho, ux, uy, uz, phi, F, a);
+        collide_cell<P>(f, g, rho, ux, uy, uz, phi, dsum, F, a);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
-            st<NTS>(&gb[at + i * ps], P::template enc_g<i>(g[i], tref_y));
+            st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
         });
         lag[idx] = P::enc_side(dphi);
         if (wall_lo || wall_hi) wall_rho[wall_idx] = P::enc_side(rho);
     }
 
     if constexpr (REDUCE && MODE != MODE_PROLOGUE) {
+        const double phid = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
         diag_terms(q, true, iy, ly, static_cast<double>(rho), static_cast<double>(ux),
-                   static_cast<double>(uy), static_cast<double>(uz), static_cast<double>(phi));
+                   static_cast<double>(uy), static_cast<double>(uz), phid);
     }
 }
 

@@ -30,10 +30,14 @@ WALLS = dict(t_hot=0.5, t_cold=-0.5)
 # Bounds after 40 steps from the conductive start.  A stored deviation carries
 # a relative rounding error of 2^-12 (binary16, scaled) or 2^-9 (bfloat16);
 # the deviations of g from w_i T_ref are ~1e-3, those of f ~1e-3 w_i, so each
-# step moves T by at most a few 1e-7 (FP16) or a few 1e-6 (BF16) per cell.
+# step moves T by at most a few 1e-8 (FP16) or a few 1e-7 (BF16) per cell.
+# Measured on MI355X: FP16 max|dT| 1.0e-6, rms 1.2e-7, mass 1.6e-9 per cell,
+# wall Nu 1.2e-4 relative; BF16 8.5e-6, 9.3e-7, 6.8e-9, 2.1e-4.  The bounds
+# leave a factor 4-5.  (Before the 16-bit arithmetic worked on deviations,
+# FP16 measured 4.9e-6, 9.8e-7 and 8.7e-4 on the same case.)
 BOUNDS_512 = {
-    "fp16": dict(max_dT=2e-5, rms_dT=2e-6, mass_per_cell=1e-9, nu_rel=1e-4),
-    "bf16": dict(max_dT=2e-4, rms_dT=2e-5, mass_per_cell=1e-7, nu_rel=1e-3),
+    "fp16": dict(max_dT=5e-6, rms_dT=5e-7, mass_per_cell=6e-9, nu_rel=6e-4),
+    "bf16": dict(max_dT=4e-5, rms_dT=4e-6, mass_per_cell=3e-8, nu_rel=1e-3),
 }
 
 
@@ -48,24 +52,26 @@ def test_512_ra1e8_16bit_storage_beside_fp64():
     assert d64["nonfinite"] == 0 and d64["step"] == steps
     phi64 = ref.get_field("phi")
     ref.close()
-    for prec, b in BOUNDS_512.items():
+    got = {}
+    for prec in BOUNDS_512:
         c = Cavity(n, n, n, precision=prec, **kw)
         c.init_conductive(3, 1.0e-3)
         c.step(steps)
         d = c.reduce()
         dT = np.abs(c.get_field("phi") - phi64)
         c.close()
-        mass = abs(d["mass"]) / n ** 3
-        rel = {k: abs(d[k] - d64[k]) / abs(d64[k]) for k in ("nu_top", "nu_bottom")}
-        print(f"{prec}: max|dT| {dT.max():.3e} rms {np.sqrt(np.mean(dT ** 2)):.3e} mass/cell {mass:.3e} "
-              f"Nu_top rel {rel['nu_top']:.3e} Nu_bottom rel {rel['nu_bottom']:.3e} "
-              f"(FP64 mass/cell {abs(d64['mass']) / n ** 3:.3e})", flush=True)
-        assert d["nonfinite"] == 0 and d["step"] == steps
-        assert dT.max() < b["max_dT"], prec
-        assert np.sqrt(np.mean(dT ** 2)) < b["rms_dT"], prec
-        assert mass < b["mass_per_cell"], prec
-        for k, v in rel.items():
-            assert v < b["nu_rel"], (prec, k)
+        m = dict(nonfinite=d["nonfinite"], step=d["step"], max_dT=float(dT.max()),
+                 rms_dT=float(np.sqrt(np.mean(dT ** 2))), mass_per_cell=abs(d["mass"]) / n ** 3,
+                 nu_rel=max(abs(d[k] - d64[k]) / abs(d64[k]) for k in ("nu_top", "nu_bottom")))
+        del dT
+        print(f"{prec}: " + " ".join(f"{k} {v:.3e}" for k, v in m.items()) +
+              f" (FP64 mass/cell {abs(d64['mass']) / n ** 3:.3e})", flush=True)
+        got[prec] = m
+    for prec, b in BOUNDS_512.items():
+        m = got[prec]
+        assert m["nonfinite"] == 0 and m["step"] == steps
+        for k, bound in b.items():
+            assert m[k] < bound, (prec, k, m[k], bound)
 
 
 # ---------------------------------------------------------------- the 64-cavity ensemble draw
@@ -107,8 +113,11 @@ def test_64_cavity_draw_members_equal_oracle(pattern):
         o.init_conductive(c["seed"], 1.0e-3)
         o.run(11)
         od = o.diag()
-        for key in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy", "mass"):
+        for key in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
             assert np.isclose(d[key], od[key], rtol=1e-11, atol=1e-14), (k, key, d[key], od[key])
+        # sum of rho' cancels to ~1e-14: another summation order moves it by
+        # N * eps * |rho'|
+        assert abs(d["mass"] - od["mass"]) < 1e-12, (k, d["mass"], od["mass"])
         o.run(1)
         s = m.state()
         for key in ("f", "g", "rho", "ux", "uy", "uz", "phi"):
