@@ -53,30 +53,75 @@ def traffic_from_profiles(workload_key: str):
     return None
 
 
-def cpu_baseline(size: int, ra: float, budget_s: float):
-    """Time the FP64 oracle (OpenMP) on a bounded sample of the workload."""
+def cpu_topology():
+    """CPU model and the physical cores (distinct core/socket pairs) among the
+    CPUs this process may run on, from lscpu."""
+    import subprocess
+
+    model, cores = None, None
+    try:
+        info = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in info.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        allowed = os.sched_getaffinity(0)
+        table = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        phys = set()
+        for line in table.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(x) if x else -1 for x in line.split(",")[:3])
+            if cpu in allowed:
+                phys.add((sock, core))
+        cores = len(phys) or None
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
+    return model, cores
+
+
+def cpu_baseline_child(size: int, ra: float, min_steps: int) -> None:
+    """Runs inside the child process of cpu_baseline: the oracle on a
+    size^3 cavity, 2 warm-up steps, then at least min_steps timed steps."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     o = oracle.OracleState(size, size, size, rayleigh=ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
     o.init_conductive(2, 1.0e-3)
     o.run(2)  # warm-up (page faults, thread pool)
-    steps = 0
     t0 = time.perf_counter()
-    while True:
-        o.run(1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 10000:
-            break
-    mlups = size ** 3 * steps / el / 1e6
+    o.run(min_steps)
+    el = time.perf_counter() - t0
+    print(json.dumps({"steps": min_steps, "seconds": el}), flush=True)
+
+
+def cpu_baseline(size: int, ra: float, min_steps: int):
+    """Times the FP64 oracle (C/OpenMP) on a size^3 sample of the workload in
+    a child process with OpenMP threads bound one per physical core
+    (OMP_PROC_BIND=close, OMP_PLACES=cores; OMP_NUM_THREADS = the physical
+    cores available, capped by the environment's OMP_NUM_THREADS)."""
+    import subprocess
+
+    model, phys = cpu_topology()
+    threads = phys or os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if cap > 0:
+        threads = min(threads, cap)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-baseline-child",
+                        f"{size},{ra!r},{min_steps}"], env=env, capture_output=True, text=True, timeout=1800)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    mlups = size ** 3 * res["steps"] / res["seconds"] / 1e6
     return {
         "value": round(mlups, 3),
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle (C/OpenMP transcription of src/), {size}^3 cavity Ra={ra:g}, "
-                  f"{steps} steps timed after 2 warm-up steps, {el:.1f} s",
+                  f"{res['steps']} steps timed after 2 warm-up steps, {res['seconds']:.1f} s; "
+                  f"{threads} OpenMP threads, OMP_PROC_BIND=close, OMP_PLACES=cores",
+        "cpu_model": model,
+        "physical_cores_available": phys,
     }
 
 
@@ -211,9 +256,11 @@ def run_serve_workload(args) -> dict:
     res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
                     hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit,
                     max_resident_cells=int(args.serve_max_cells), progress_s=30.0, pattern=args.serve_pattern)
-    log(f"serve: {st['completed']} completed, {st['unfinished']} unfinished, {st['pending']} pending, "
-        f"{st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
-        f"p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; peak resident {st['peak_resident']}")
+    log(f"serve: {st['arrived']} arrived, {st['completed']} completed, {st['unfinished']} unfinished, "
+        f"{st['pending']} pending, {st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
+        f"completed p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; all arrived (censored at the end) "
+        f"p50 {st['p50_all_s']:.2f} s p99 {st['p99_all_s']:.2f} s; mean queued {st['mean_queue_s']:.2f} s, "
+        f"resident {st['mean_service_s']:.2f} s; peak resident {st['peak_resident']}")
     # sequential single-cavity rate over the same request sizes (bounded sample)
     tot, upd = 0.0, 0.0
     for r in reqs[: args.serve_seq_sample]:
@@ -302,6 +349,93 @@ def run_precision_workload(args) -> dict:
     return res
 
 
+def run_precision_seeds_workload(args) -> dict:
+    """Seed ensemble of the precision study: `seeds` cavities of one size, Ra
+    and precision, identical but for the seed of the initial perturbation
+    (seeds 1..N), advanced together in one ensemble launch per step.  Nu of
+    every member is sampled every `sample` steps; each member's Nu is
+    time-averaged over (avg_from, steps].  Comparing the seed-averaged values
+    of two precisions separates a storage bias from the divergence of chaotic
+    trajectories, which the average over independent seeds removes."""
+    import numpy as np
+
+    from lbm_ddf_amd import Ensemble
+
+    L, prec = args.size, args.precision
+    e = Ensemble(prec)
+    slots = [e.add(L, L, L, seed=k + 1, amp=1.0e-3, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+             for k in range(args.seeds)]
+    e.set_tuning(*args.seed_tuning)
+    series = {s: [] for s in slots}
+    t0 = time.perf_counter()
+    done, next_log = 0, 0.0
+    while done < args.steps:
+        n = min(args.sample, args.steps - done)
+        diags = e.run_diag(n)
+        done += n
+        for s in slots:
+            d = diags[s]
+            series[s].append((d["step"], d["nu_bulk"], d["nu_top"], d["nu_bottom"], d["kinetic_energy"], d["mass"],
+                              d["nonfinite"]))
+        el = time.perf_counter() - t0
+        if el >= next_log:
+            nb = np.mean([series[s][-1][1] for s in slots])
+            log(f"precision seeds {prec}: step {done}/{args.steps}, {el:.0f} s, mean Nu_bulk {nb:.4f}")
+            next_log = el + 30.0
+    wall = time.perf_counter() - t0
+    e.close()
+    members = []
+    for s in slots:
+        a = np.array(series[s])
+        sel = a[a[:, 0] > args.avg_from]
+        members.append({"seed": s + 1, "nu_bulk": float(sel[:, 1].mean()), "nu_top": float(sel[:, 2].mean()),
+                        "nu_bottom": float(sel[:, 3].mean()), "kinetic_energy": float(sel[:, 4].mean()),
+                        "mass_drift_per_cell": float((a[-1, 5] - a[0, 5]) / L ** 3),
+                        "nonfinite": float(a[:, 6].max()), "samples": int(len(sel))})
+    return {"precision": prec, "size": L, "ra": args.ra, "seeds": args.seeds, "steps": args.steps,
+            "avg_from": args.avg_from, "sample": args.sample,
+            "mlups": args.seeds * L ** 3 * args.steps / wall / 1e6, "members": members,
+            "series": {str(s + 1): series[s] for s in slots}}
+
+
+def precision_seeds_report(size: int, ra: float, out_dir: str) -> dict:
+    """Seed-averaged time-mean Nu of each precision against FP64 (files the
+    precision-seeds workload wrote), with 95% confidence intervals of the
+    differences from the seed-to-seed spread (Welch)."""
+    import numpy as np
+    from scipy.stats import t as student_t
+
+    d = Path(out_dir)
+    runs = {}
+    for prec in ("fp64", "fp16", "bf16"):
+        f = d / f"precision_seeds_{prec}_{size}_{ra:g}.json"
+        if f.exists():
+            runs[prec] = json.loads(f.read_text())
+    out = {}
+    if "fp64" not in runs:
+        return {"error": "no FP64 seed run in " + str(d)}
+    ref = runs["fp64"]
+    for prec, r in runs.items():
+        rec = {"seeds": r["seeds"], "steps": r["steps"], "avg_from": r["avg_from"], "mlups": r["mlups"],
+               "mass_drift_per_cell_mean": float(np.mean([m["mass_drift_per_cell"] for m in r["members"]])),
+               "nonfinite_members": int(sum(1 for m in r["members"] if m["nonfinite"] > 0))}
+        for k in ("nu_bulk", "nu_top", "nu_bottom"):
+            x = np.array([m[k] for m in r["members"]])
+            y = np.array([m[k] for m in ref["members"]])
+            mx, my = x.mean(), y.mean()
+            vx, vy = x.var(ddof=1) / len(x), y.var(ddof=1) / len(y)
+            se = float(np.sqrt(vx + vy))
+            dof = (vx + vy) ** 2 / (vx ** 2 / (len(x) - 1) + vy ** 2 / (len(y) - 1)) if vx + vy > 0 else 1.0
+            half = float(student_t.ppf(0.975, dof) * se)
+            rec[k] = {"mean": float(mx), "seed_sd": float(x.std(ddof=1)), "ci95_mean": float(
+                student_t.ppf(0.975, len(x) - 1) * x.std(ddof=1) / np.sqrt(len(x)))}
+            if prec != "fp64":
+                rec[k].update({"diff_vs_fp64": float(mx - my), "diff_ci95": half,
+                               "rel_diff_vs_fp64": float((mx - my) / my), "rel_diff_ci95": half / abs(my)})
+        out[prec] = rec
+    return out
+
+
 def precision_report(size: int, avg_from: int, out_dir: str) -> dict:
     """Compares the 16-bit runs of the precision workload with the FP64 run
     (files that --workload precision wrote to out_dir)."""
@@ -353,15 +487,18 @@ def main() -> int:
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp16", "bf16"])
     ap.add_argument("--pattern", default="ab", choices=["ab", "aa"],
                     help="population storage: two buffers (ab) or one, updated in place (aa)")
-    ap.add_argument("--cpu-size", type=int, default=128)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-size", type=int, default=256,
+                    help="CPU baseline sample: cube side (the 512^3 workload needs ~125 GB of host memory)")
+    ap.add_argument("--cpu-steps", type=int, default=50, help="CPU baseline sample: timed oracle steps")
+    ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage leg of the default workload")
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
     ap.add_argument("--workload", default="single",
-                    choices=["single", "ensemble", "serve", "precision", "precision-report", "config1"],
+                    choices=["single", "ensemble", "serve", "precision", "precision-report", "precision-seeds",
+                             "precision-seeds-report", "config1"],
                     help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--config1-steps", type=int, default=1000)
@@ -386,6 +523,10 @@ def main() -> int:
     ap.add_argument("--sample", type=int, default=100, help="precision workload: steps between Nu samples")
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
+    ap.add_argument("--seeds", type=int, default=16, help="precision-seeds workload: members (seeds 1..N)")
+    ap.add_argument("--seed-tuning", type=int, nargs=5, default=[256, 8, 8, 7, 1],
+                    metavar=("BX", "BPC", "PAIRS", "FLAGS", "CPT"),
+                    help="precision-seeds workload: ensemble launch shape")
     ap.add_argument("--out-dir", default=str(ROOT / "bench_out"),
                     help="precision workload: where its series and T snapshots are written and read "
                          "back; ensemble workload: the tuning cache")
@@ -393,6 +534,11 @@ def main() -> int:
                     help="only run a plain device copy of GIB GiB (5 reps) and print its GB/s; "
                          "the HBM-counter calibration run")
     args = ap.parse_args()
+
+    if args.cpu_baseline_child:
+        size, ra, steps = args.cpu_baseline_child.split(",")
+        cpu_baseline_child(int(size), float(ra), int(steps))
+        return 0
 
     if args.copy_calibration > 0:
         from lbm_ddf_amd import capi
@@ -409,6 +555,18 @@ def main() -> int:
 
     if args.workload == "precision-report":
         print(json.dumps(precision_report(args.size, args.avg_from, args.out_dir), indent=1), flush=True)
+        return 0
+
+    if args.workload == "precision-seeds":
+        res = run_precision_seeds_workload(args)
+        Path(args.out_dir).mkdir(parents=True, exist_ok=True)
+        (Path(args.out_dir) / f"precision_seeds_{args.precision}_{args.size}_{args.ra:g}.json").write_text(
+            json.dumps(res))
+        print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
+        return 0
+
+    if args.workload == "precision-seeds-report":
+        print(json.dumps(precision_seeds_report(args.size, args.ra, args.out_dir), indent=1), flush=True)
         return 0
 
     if args.workload == "precision":
@@ -448,7 +606,14 @@ def main() -> int:
                                    "stop |dNu|<1e-3 per 1000 steps or 50k steps, HBM budget "
                                    f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit,
                        "pattern": args.serve_pattern},
-            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "serve": res,
+            "p50_s": st["p50_s"], "p99_s": st["p99_s"],
+            "p50_all_arrived_s": st["p50_all_s"], "p99_all_arrived_s": st["p99_all_s"],
+            "latency_note": "p50_s/p99_s: completed requests only; *_all_arrived_s: every arrived request, "
+                            "unfinished and pending ones counted as finishing at the end of the run "
+                            "(right-censored, so lower bounds)",
+            "queue_share_of_latency": (st["mean_queue_s"] / (st["mean_queue_s"] + st["mean_service_s"])
+                                       if st["completed"] else None),
+            "serve": res,
         }), flush=True)
         return 0
 
@@ -562,8 +727,8 @@ def main() -> int:
                     "step": d["step"]},
     }
     if not args.no_cpu:
-        log(f"CPU baseline: oracle on {args.cpu_size}^3 for ~{args.cpu_seconds:.0f} s")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.ra, args.cpu_seconds)
+        log(f"CPU baseline: oracle on {args.cpu_size}^3 for {args.cpu_steps} steps")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.ra, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
     if prec == "fp64" and not args.no_fp16:

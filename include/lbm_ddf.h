@@ -261,7 +261,8 @@ int lbm_ensemble_step_timed(lbm_ensemble *ens, int64_t nsteps, double *elapsed_m
  * out[slot] receives them (step = the cavity's completed step).  cap >= slots. */
 int lbm_ensemble_run_diag(lbm_ensemble *ens, int64_t nsteps, lbm_diag *out, int32_t cap);
 /* block_x (64, 128, 256) threads per block, blocks_per_cu of the persistent
- * grid, graph_pairs step pairs per graph replay (0: no graph), flags 0, 3
+ * grid (0: no persistent grid, one block per tile), graph_pairs step pairs
+ * per graph replay (0: no graph), flags 0, 3
  * (non-temporal hints), 4 (XCD-contiguous tile order) or 7 (both),
  * cells_per_thread (1, 2, 4): a tile is block_x * cells_per_thread
  * consecutive cells of one cavity.  Results do not depend on the choice. */

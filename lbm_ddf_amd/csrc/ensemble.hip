@@ -12,7 +12,9 @@
 //     parities, and a tile map gives each tile's cavity;
 //   * a persistent grid (blocks_per_cu x CUs blocks) that strides over the
 //     tiles, so the grid shape -- and a captured hipGraph -- does not depend
-//     on which cavities are resident;
+//     on which cavities are resident; or (blocks_per_cu = 0) one block per
+//     tile, dispatched by the hardware like the single-cavity kernel, with
+//     the graph re-captured when the tile count changes;
 //   * hipGraph replay of step pairs (parity 0, 1): the table is rewritten
 //     between replays, never inside one;
 //   * per-cavity diagnostics folded into a step (per-tile partials, then one
@@ -89,7 +91,8 @@ struct lbm_ensemble {
     lbm::EnsCavity *cav_host = nullptr;
     uint32_t *ctrl_host = nullptr;
     size_t host_cap = 0;
-    uint32_t ncav = 0;  // resident cavities in the table
+    uint32_t ncav = 0;    // resident cavities in the table
+    uint32_t ntiles = 0;  // tiles in the table
     // tuning
     int bx = 128;
     int blocks_per_cu = 8;
@@ -112,7 +115,11 @@ using lbm::ctx_fail;
         if (e_ != hipSuccess) return ctx_fail(LBM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int grid_of(const lbm_ensemble *e) { return std::max(1, e->blocks_per_cu * e->num_cus); }
+// blocks_per_cu = 0: one block per tile of the uploaded table
+int grid_of(const lbm_ensemble *e) {
+    if (e->blocks_per_cu == 0) return static_cast<int>(std::max<uint32_t>(1, e->ntiles));
+    return std::max(1, e->blocks_per_cu * e->num_cus);
+}
 
 void drop_graph(lbm_ensemble *e) {
     if (e->graph) {
@@ -203,7 +210,10 @@ int upload_tables(lbm_ensemble *e) {
         drop_graph(e);  // the graph captured the old map pointer
     }
     if (tiles) ENS_HIP(hipMemcpy(e->tmap_dev, e->tmap_host.data(), sizeof(uint32_t) * tiles, hipMemcpyHostToDevice));
+    // a graph of one block per tile holds the old tile count in its grid
+    if (e->blocks_per_cu == 0 && static_cast<uint32_t>(tiles) != e->ntiles) drop_graph(e);
     e->ncav = ncav;
+    e->ntiles = static_cast<uint32_t>(tiles);
     e->ctrl_host[0] = ncav;
     e->ctrl_host[1] = static_cast<uint32_t>(tiles);
     e->ctrl_host[2] = static_cast<uint32_t>(e->cpt);
@@ -470,8 +480,8 @@ int lbm_ensemble_set_tuning(lbm_ensemble *e, int32_t block_x, int32_t blocks_per
                             int32_t flags, int32_t cells_per_thread) {
     if (!e) return ctx_fail(LBM_E_INVALID, "NULL ensemble");
     if (block_x != 64 && block_x != 128 && block_x != 256) return ctx_fail(LBM_E_INVALID, "block_x must be 64, 128 or 256");
-    if (blocks_per_cu < 1 || blocks_per_cu > 64 || graph_pairs < 0 || graph_pairs > 256)
-        return ctx_fail(LBM_E_INVALID, "blocks_per_cu in 1..64, graph_pairs in 0..256");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64 || graph_pairs < 0 || graph_pairs > 256)
+        return ctx_fail(LBM_E_INVALID, "blocks_per_cu in 0..64 (0: one block per tile), graph_pairs in 0..256");
     if (flags != 0 && flags != 3 && flags != 4 && flags != 7) return ctx_fail(LBM_E_INVALID, "flags must be 0, 3, 4 or 7");
     if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
         return ctx_fail(LBM_E_INVALID, "cells_per_thread must be 1, 2 or 4");
@@ -531,7 +541,7 @@ int lbm_ensemble_autotune(lbm_ensemble *e, int32_t steps, const char *cache_path
     // Stage 1: launch shapes at one cell per thread.
     std::vector<Cand> cands;
     for (int bx : {64, 128, 256})
-        for (int bpc : {4, 8, 16})
+        for (int bpc : {0, 4, 8, 16})
             for (int fl : {0, 3, 4, 7}) {
                 Cand c{bx, bpc, fl, 1, 0.0};
                 if (int rc = time_short(c)) return rc;

@@ -61,3 +61,39 @@ def test_batch_means_interval(nbatch, t975):
     assert h == pytest.approx(t975 * means.std(ddof=1) / np.sqrt(nbatch), rel=1e-5)
     # independent normal samples: the interval covers the true mean 0 here
     assert abs(m) < h
+
+
+def test_cpu_baseline_runs_bound_oracle_in_a_child():
+    """The CPU baseline times the oracle in a child process with OpenMP bound
+    one thread per physical core, and reports the model and core count."""
+    b = load_bench()
+    r = b.cpu_baseline(12, 1e5, 3)
+    assert r["kind"] == "port" and r["unit"] == "MLUPS" and r["value"] > 0
+    assert r["cores"] >= 1 and "OMP_PROC_BIND=close" in r["sample"] and "12^3" in r["sample"]
+    assert "cpu_model" in r and "physical_cores_available" in r
+
+
+def test_precision_seeds_report_differences_and_intervals(tmp_path):
+    """The seed report compares seed-averaged time means with FP64 and gives
+    Welch 95% intervals from the seed-to-seed spread."""
+    import json
+
+    b = load_bench()
+    rng = np.random.default_rng(0)
+
+    def run(prec, shift):
+        members = [{"seed": k + 1, "nu_bulk": 8.0 + shift + rng.normal(0, 0.2), "nu_top": 8.0 + shift,
+                    "nu_bottom": 8.0 + shift, "kinetic_energy": 1e-3, "mass_drift_per_cell": 1e-6,
+                    "nonfinite": 0.0, "samples": 100} for k in range(16)]
+        return {"precision": prec, "size": 32, "ra": 1e6, "seeds": 16, "steps": 1000, "avg_from": 500,
+                "sample": 100, "mlups": 1.0, "members": members}
+
+    for prec, shift in (("fp64", 0.0), ("fp16", 0.04), ("bf16", 1.0)):
+        (tmp_path / f"precision_seeds_{prec}_32_1e+06.json").write_text(json.dumps(run(prec, shift)))
+    r = b.precision_seeds_report(32, 1e6, str(tmp_path))
+    assert set(r) == {"fp64", "fp16", "bf16"}
+    assert r["fp16"]["nu_top"]["diff_vs_fp64"] == pytest.approx(0.04)
+    assert r["fp16"]["nu_top"]["rel_diff_vs_fp64"] == pytest.approx(0.005)
+    assert r["bf16"]["nu_bulk"]["diff_vs_fp64"] - r["bf16"]["nu_bulk"]["diff_ci95"] > 0.5
+    # 16 seeds with sd 0.2 each: a Welch half-width near 2.04 * 0.2 * sqrt(2/16)
+    assert 0.08 < r["fp16"]["nu_bulk"]["diff_ci95"] < 0.25

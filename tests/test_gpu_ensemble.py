@@ -54,10 +54,11 @@ def assert_member_equals(e, slot, cav, precision="fp64", pattern="ab"):
     return k
 
 
-@pytest.mark.parametrize("graph_pairs,cpt", [(0, 1), (4, 1), (4, 2), (0, 4)])
-def test_members_equal_standalone_runs(graph_pairs, cpt):
+@pytest.mark.parametrize("graph_pairs,cpt,bpc", [(0, 1, 4), (4, 1, 4), (4, 2, 4), (0, 4, 4), (4, 1, 0), (0, 2, 0)])
+def test_members_equal_standalone_runs(graph_pairs, cpt, bpc):
+    """bpc: blocks per CU of the persistent grid, 0 for one block per tile."""
     e, slots = build()
-    e.set_tuning(128, 4, graph_pairs, 7 if graph_pairs else 0, cpt)
+    e.set_tuning(128, bpc, graph_pairs, 7 if graph_pairs else 0, cpt)
     e.step(37)
     for s, cav in zip(slots, CAVS):
         # prologue (collides step 1) + 37 launches
@@ -266,5 +267,23 @@ def test_member_cannot_be_stepped_or_destroyed_alone():
     assert m.lib.lbm_destroy(m.h) == -4  # LBM_E_STATE
     e.step(3)
     for s, cav in zip(slots, CAVS[:2]):
-        assert assert_member_equals(e, s, cav) == 9  # (prologue + 4) + (prologue + 3)
+        assert assert_member_equals(e, s, cav) == 8  # prologue + 4 + 3: the refused calls read nothing
+    e.close()
+
+
+def test_one_block_per_tile_across_joins_and_graph_replays():
+    """With one block per tile the launch grid follows the tile count: a
+    join or a removal between graph replays re-captures the graph, and every
+    member stays exact."""
+    e, slots = build(cavs=CAVS[:3])
+    e.set_tuning(64, 0, 2, 7, 1)  # 4 steps per replay
+    e.step(9)
+    s4 = e.add(*CAVS[5]["shape"], seed=CAVS[5]["seed"], amp=AMP, rayleigh=CAVS[5]["rayleigh"],
+               prandtl=CAVS[5]["prandtl"], **WALLS)
+    e.step(8)
+    e.remove(slots[1])
+    e.step(12)
+    assert assert_member_equals(e, slots[0], CAVS[0]) == 30  # prologue + 9 + 8 + 12
+    assert assert_member_equals(e, slots[2], CAVS[2]) == 30
+    assert assert_member_equals(e, s4, CAVS[5]) == 21  # prologue + 8 + 12
     e.close()
