@@ -176,6 +176,7 @@ def run_ensemble_workload(args) -> dict:
     cavs = ensemble_cavities()
     cells = sum(c["lx"] * c["ly"] * c["lz"] for c in cavs)
     out = {"cavities": len(cavs), "cells": cells, "precisions": {}}
+    ref_seq = None  # reference execution structure, FP64 (measured once)
     for prec in args.ensemble_precisions.split(","):
         e = Ensemble(prec)
         for c in cavs:
@@ -183,13 +184,18 @@ def run_ensemble_workload(args) -> dict:
                   prandtl=c["prandtl"], t_hot=0.5, t_cold=-0.5, pattern=args.ensemble_pattern)
         e.step(2)
         t = time.perf_counter()
-        if args.tuning_cache:
-            Path(args.out_dir).mkdir(parents=True, exist_ok=True)
-        e.autotune(8, str(Path(args.out_dir) / "ensemble_tuning.txt") if args.tuning_cache else None)
-        bx, bpc, gp, fl, cpt = e.tuning
-        e.set_tuning(bx, bpc, 16, fl, cpt)
-        log(f"ensemble {prec}: autotune {time.perf_counter() - t:.1f} s -> block {bx}, {bpc} blocks/CU, "
-            f"flags {fl}, {cpt} cells/thread")
+        if args.ensemble_tuning:
+            bx, bpc, gp, fl, cpt = args.ensemble_tuning
+            e.set_tuning(bx, bpc, gp, fl, cpt)
+            log(f"ensemble {prec}: fixed launch shape block {bx}, {bpc} blocks/CU, flags {fl}, {cpt} cells/thread")
+        else:
+            if args.tuning_cache:
+                Path(args.out_dir).mkdir(parents=True, exist_ok=True)
+            e.autotune(8, str(Path(args.out_dir) / "ensemble_tuning.txt") if args.tuning_cache else None)
+            bx, bpc, gp, fl, cpt = e.tuning
+            e.set_tuning(bx, bpc, 16, fl, cpt)
+            log(f"ensemble {prec}: autotune {time.perf_counter() - t:.1f} s -> block {bx}, "
+                f"{bpc} blocks/CU (0: one block per tile), flags {fl}, {cpt} cells/thread")
         e.step(32)
         e.count()
         t0 = time.perf_counter()
@@ -200,10 +206,10 @@ def run_ensemble_workload(args) -> dict:
         bad = sum(1 for d in diags.values() if d["nonfinite"] != 0)
         e.close()
         log(f"ensemble {prec}: {args.ensemble_steps} steps in {wall:.2f} s -> {agg:.0f} MLUPS aggregate")
-        # sequential single-cavity runs, bounded sample of steps per cavity
-        seq_steps = args.sequential_steps
+        # sequential single-cavity runs over the same step window as the ensemble
+        seq_steps = args.sequential_steps or args.ensemble_steps
         tot = 0.0
-        for c in cavs:
+        for c in (cavs if seq_steps > 0 else []):
             cv = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], t_hot=0.5,
                         t_cold=-0.5, precision=prec, pattern=args.ensemble_pattern)
             cv.init_conductive(c["seed"], 1.0e-3)
@@ -214,31 +220,40 @@ def run_ensemble_workload(args) -> dict:
             cv.synchronize()
             tot += time.perf_counter() - t1
             cv.close()
-        seq = cells * seq_steps / tot / 1e6
-        log(f"sequential {prec}: {seq:.0f} MLUPS ({seq_steps} steps per cavity, {tot:.2f} s)")
-        # the reference's execution structure (stage launches + 3 syncs per step), FP64 only
-        ref_seq = None
-        if prec == "fp64":
+        seq = cells * seq_steps / tot / 1e6 if tot > 0 else None
+        if seq:
+            log(f"sequential {prec}: {seq:.0f} MLUPS ({seq_steps} steps per cavity, {tot:.2f} s)")
+        # the reference's execution structure (stage launches + 3 syncs per
+        # step) over the same window; the stage kernels are FP64 only, as the
+        # reference is, so a 16-bit ensemble is compared with the FP64 figure
+        if ref_seq is None and args.staged_steps >= 0:
             tot_r, upd_r = 0.0, 0.0
             for c in cavs:
                 cv = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], t_hot=0.5,
                             t_cold=-0.5)
                 cv.init_conductive(c["seed"], 1.0e-3)
                 cv.step_staged(2)
+                staged = args.staged_steps or args.ensemble_steps
                 t1 = time.perf_counter()
-                cv.step_staged(args.staged_steps)
+                cv.step_staged(staged)
                 tot_r += time.perf_counter() - t1
-                upd_r += c["lx"] * c["ly"] * c["lz"] * args.staged_steps
+                upd_r += c["lx"] * c["ly"] * c["lz"] * staged
                 cv.close()
             ref_seq = upd_r / tot_r / 1e6
-            log(f"sequential {prec}, reference structure: {ref_seq:.0f} MLUPS")
+            log(f"sequential fp64, reference structure: {ref_seq:.0f} MLUPS ({staged} steps per cavity)")
         out["precisions"][prec] = {
             "aggregate_mlups": round(agg, 1), "ms_per_step": round(wall / args.ensemble_steps * 1e3, 4),
-            "event_ms_per_step": round(ms / args.ensemble_steps, 4), "sequential_mlups": round(seq, 1),
-            "speedup_vs_sequential": round(agg / seq, 2),
+            "event_ms_per_step": round(ms / args.ensemble_steps, 4),
+            "sequential_mlups": round(seq, 1) if seq else None,
+            "speedup_vs_sequential": round(agg / seq, 2) if seq else None,
             "sequential_reference_structure_mlups": round(ref_seq, 1) if ref_seq else None,
+            "reference_structure_precision": "fp64",
             "speedup_vs_reference_structure": round(agg / ref_seq, 2) if ref_seq else None,
-            "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "cells_per_thread": cpt, "graph_steps": 32},
+            "roofline_frac": round(BYTES_PER_CELL[prec] * cells / (ms / args.ensemble_steps * 1e-3) / 1e9
+                                   / HBM_PEAK_GBS, 4),
+            "launch": {"block_x": bx, "blocks_per_cu": bpc, "flags": fl, "cells_per_thread": cpt,
+                       "graph_steps": 2 * (gp if args.ensemble_tuning else 16),
+                       "autotuned": not args.ensemble_tuning},
             "nonfinite_cavities": bad,
         }
     return out
@@ -301,6 +316,16 @@ def batch_means(x, nbatch: int = 10):
     return float(x.mean()), float(half)
 
 
+def precision_shape(args):
+    """(LX, LY, LZ) of the precision workload: --shape, else a --size cube."""
+    return tuple(args.shape) if args.shape else (args.size, args.size, args.size)
+
+
+def shape_tag(args) -> str:
+    lx, ly, lz = precision_shape(args)
+    return f"{lx}" if lx == ly == lz else f"{lx}x{ly}x{lz}"
+
+
 def run_precision_workload(args) -> dict:
     """One precision of the reduced-precision study: L^3 cavity at Ra, the
     diagnostics folded in every `sample` steps, Nu time-averaged over
@@ -310,13 +335,15 @@ def run_precision_workload(args) -> dict:
 
     from lbm_ddf_amd import Cavity
 
-    L, prec = args.size, args.precision
-    c = Cavity(L, L, L, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
-    c.init_conductive(3, 1.0e-3)
+    prec = args.precision
+    lx, ly, lz = precision_shape(args)
+    n = lx * ly * lz
+    c = Cavity(lx, ly, lz, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
+    c.init_conductive(args.seed, 1.0e-3)
     if args.autotune > 0:
         c.autotune(args.autotune)
         # the autotuner advanced the state; restart from the same seeded state
-        c.init_conductive(3, 1.0e-3)
+        c.init_conductive(args.seed, 1.0e-3)
     recs = []
     t0 = time.perf_counter()
     done = 0
@@ -333,13 +360,13 @@ def run_precision_workload(args) -> dict:
     snap = phi[::args.snapshot_stride, ::args.snapshot_stride, ::args.snapshot_stride].astype(np.float64)
     out_dir = Path(args.out_dir)
     out_dir.mkdir(parents=True, exist_ok=True)
-    np.save(out_dir / f"precision_T_{prec}_{L}.npy", snap)
+    np.save(out_dir / f"precision_T_{prec}_{shape_tag(args)}.npy", snap)
     c.close()
     sel = [r for r in recs if r["step"] > args.avg_from]
-    res = {"precision": prec, "size": L, "ra": args.ra, "steps": args.steps, "samples": len(sel),
-           "mlups": L ** 3 * args.steps / wall / 1e6,
+    res = {"precision": prec, "shape": [lx, ly, lz], "ra": args.ra, "seed": args.seed, "steps": args.steps,
+           "samples": len(sel), "mlups": n * args.steps / wall / 1e6,
            "mass_start": recs[0]["mass"], "mass_end": recs[-1]["mass"],
-           "mass_drift_per_cell": (recs[-1]["mass"] - recs[0]["mass"]) / L ** 3,
+           "mass_drift_per_cell": (recs[-1]["mass"] - recs[0]["mass"]) / n,
            "nonfinite": recs[-1]["nonfinite"]}
     for k in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
         m, h = batch_means([r[k] for r in sel])
@@ -436,7 +463,7 @@ def precision_seeds_report(size: int, ra: float, out_dir: str) -> dict:
     return out
 
 
-def precision_report(size: int, avg_from: int, out_dir: str) -> dict:
+def precision_report(tag: str, avg_from: int, out_dir: str) -> dict:
     """Compares the 16-bit runs of the precision workload with the FP64 run
     (files that --workload precision wrote to out_dir)."""
     import numpy as np
@@ -444,15 +471,15 @@ def precision_report(size: int, avg_from: int, out_dir: str) -> dict:
     d = Path(out_dir)
     runs = {}
     for prec in ("fp64", "fp16", "bf16"):
-        f = d / f"precision_{prec}_{size}.json"
+        f = d / f"precision_{prec}_{tag}.json"
         if f.exists():
             runs[prec] = json.loads(f.read_text())
     if "fp64" not in runs:
         raise SystemExit("precision-report: the FP64 run is missing")
     ref = runs["fp64"]
     a_ref = np.array(ref["series"])
-    t_ref = np.load(d / f"precision_T_fp64_{size}.npy")
-    out = {"size": size, "ra": ref["ra"], "steps": ref["steps"], "avg_from": avg_from, "fp64": {
+    t_ref = np.load(d / f"precision_T_fp64_{tag}.npy")
+    out = {"shape": tag, "ra": ref["ra"], "steps": ref["steps"], "avg_from": avg_from, "fp64": {
         k: ref[k] for k in ("nu_bulk", "nu_top", "nu_bottom", "mass_drift_per_cell", "mlups")}}
     for prec, r in runs.items():
         if prec == "fp64":
@@ -462,10 +489,12 @@ def precision_report(size: int, avg_from: int, out_dir: str) -> dict:
         rec = {k: r[k] for k in ("nu_bulk", "nu_top", "nu_bottom", "mass_drift_per_cell", "mlups")}
         for j, k in ((1, "nu_bulk"), (2, "nu_top"), (3, "nu_bottom")):
             m = a_ref[:n, 0] > avg_from
-            rec[f"{k}_time_avg_rel_err"] = float(abs(a[:n][m, j].mean() / a_ref[:n][m, j].mean() - 1.0))
+            rel = float(a[:n][m, j].mean() / a_ref[:n][m, j].mean() - 1.0)
+            rec[f"{k}_time_avg_rel_err"] = abs(rel)
+            rec[f"{k}_time_avg_rel_diff"] = rel
             rec[f"{k}_max_abs_diff_series"] = float(np.max(np.abs(a[:n, j] - a_ref[:n, j])))
         rec["speedup_vs_fp64"] = r["mlups"] / ref["mlups"]
-        tf = d / f"precision_T_{prec}_{size}.npy"
+        tf = d / f"precision_T_{prec}_{tag}.npy"
         if tf.exists():
             t = np.load(tf)
             rec["max_abs_dT_snapshot"] = float(np.max(np.abs(t - t_ref)))
@@ -505,10 +534,17 @@ def main() -> int:
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
     ap.add_argument("--ensemble-pattern", default="ab", choices=["ab", "aa"],
                     help="ensemble members (and the sequential comparison) with two buffers or one in place")
-    ap.add_argument("--sequential-steps", type=int, default=300)
+    ap.add_argument("--sequential-steps", type=int, default=0,
+                    help="ensemble workload: steps per cavity of the sequential runs (0: --ensemble-steps, "
+                         "-1: skip)")
+    ap.add_argument("--ensemble-tuning", type=int, nargs=5, default=None,
+                    metavar=("BX", "BPC", "PAIRS", "FLAGS", "CPT"),
+                    help="ensemble workload: fixed launch shape instead of the autotuner")
     ap.add_argument("--tuning-cache", action="store_true",
                     help="ensemble workload: reuse/record the autotuner's choice in <out-dir>/ensemble_tuning.txt")
-    ap.add_argument("--staged-steps", type=int, default=20)
+    ap.add_argument("--staged-steps", type=int, default=0,
+                    help="ensemble workload: steps per cavity in the reference's execution structure "
+                         "(0: --ensemble-steps, -1: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-arrivals", type=float, default=60.0, help="seconds during which requests arrive")
     ap.add_argument("--serve-time-limit", type=float, default=120.0)
@@ -524,6 +560,9 @@ def main() -> int:
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
     ap.add_argument("--seeds", type=int, default=16, help="precision-seeds workload: members (seeds 1..N)")
+    ap.add_argument("--shape", type=int, nargs=3, default=None, metavar=("LX", "LY", "LZ"),
+                    help="precision workload: cavity shape (default: a --size cube)")
+    ap.add_argument("--seed", type=int, default=3, help="precision workload: seed of the initial perturbation")
     ap.add_argument("--seed-tuning", type=int, nargs=5, default=[256, 8, 8, 7, 1],
                     metavar=("BX", "BPC", "PAIRS", "FLAGS", "CPT"),
                     help="precision-seeds workload: ensemble launch shape")
@@ -554,7 +593,7 @@ def main() -> int:
         return 2
 
     if args.workload == "precision-report":
-        print(json.dumps(precision_report(args.size, args.avg_from, args.out_dir), indent=1), flush=True)
+        print(json.dumps(precision_report(shape_tag(args), args.avg_from, args.out_dir), indent=1), flush=True)
         return 0
 
     if args.workload == "precision-seeds":
@@ -572,7 +611,7 @@ def main() -> int:
     if args.workload == "precision":
         res = run_precision_workload(args)
         Path(args.out_dir).mkdir(parents=True, exist_ok=True)
-        (Path(args.out_dir) / f"precision_{args.precision}_{args.size}.json").write_text(json.dumps(res))
+        (Path(args.out_dir) / f"precision_{args.precision}_{shape_tag(args)}.json").write_text(json.dumps(res))
         print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
         return 0
 

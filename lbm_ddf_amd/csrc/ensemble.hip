@@ -95,7 +95,7 @@ struct lbm_ensemble {
     uint32_t ntiles = 0;  // tiles in the table
     // tuning
     int bx = 128;
-    int blocks_per_cu = 8;
+    int blocks_per_cu = 0;  // one block per tile (measured faster than any persistent grid)
     int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
     int flags = 7;        // launch variant of the plain step (0, 3, 4, 7)
     int cpt = 1;          // cells per thread (1, 2, 4)
