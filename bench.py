@@ -337,7 +337,7 @@ def run_precision_workload(args) -> dict:
 
     prec = args.precision
     lx, ly, lz = precision_shape(args)
-    n = lx * ly * lz
+    ncell = lx * ly * lz
     c = Cavity(lx, ly, lz, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=prec)
     c.init_conductive(args.seed, 1.0e-3)
     if args.autotune > 0:
@@ -364,9 +364,9 @@ def run_precision_workload(args) -> dict:
     c.close()
     sel = [r for r in recs if r["step"] > args.avg_from]
     res = {"precision": prec, "shape": [lx, ly, lz], "ra": args.ra, "seed": args.seed, "steps": args.steps,
-           "samples": len(sel), "mlups": n * args.steps / wall / 1e6,
+           "samples": len(sel), "mlups": ncell * args.steps / wall / 1e6,
            "mass_start": recs[0]["mass"], "mass_end": recs[-1]["mass"],
-           "mass_drift_per_cell": (recs[-1]["mass"] - recs[0]["mass"]) / n,
+           "mass_drift_per_cell": (recs[-1]["mass"] - recs[0]["mass"]) / ncell,
            "nonfinite": recs[-1]["nonfinite"]}
     for k in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
         m, h = batch_means([r[k] for r in sel])
