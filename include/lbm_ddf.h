@@ -70,7 +70,7 @@ extern "C" {
 #define LBM_PATTERN_AA 1 /* one buffer per distribution, updated in place by
                             alternating even/odd passes: half the population
                             memory; FP64 results equal A/B bit for bit (16-bit
-                            storage rounds g against other rows: close) */
+                            storage rounds differently: close) */
 
 /* Physical and lattice parameters: the macros of src/parameters.h:8-26
  * made runtime.  beta <= 0 selects the reference default 0.1/(5*LY).
