@@ -27,10 +27,11 @@
 // the A/B kernel, so FP64 results are the same bit for bit; no wall-rho side
 // array is needed.
 //
-// 16-bit storage encodes g with the reference temperature of the row of the
-// cell that will read the slot as its own (the destination cell), so every
-// read decodes with the reader's own row, and layout N has the same encoding
-// as the A/B scheme's reference state.
+// 16-bit storage encodes g with the reference temperature of the writer's
+// row, as the A/B scheme does, so the stored deviations stay small: a slot
+// holding a value streamed from inside the domain decodes with its source
+// row, a wall value with the cell's own row (writer_row in tref.hpp); both
+// passes and layout N of the reference state follow that rule.
 #pragma once
 
 #include "step_kernel.hpp"
@@ -55,11 +56,11 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     S *gs = static_cast<S *>(a.gb);
     const T t0 = static_cast<T>(a.t0);
     // 16-bit g storage: this row's reference temperature and the offsets of
-    // the rows that read what this cell writes (ignored in FP64).  In
+    // the rows that wrote what this cell reads (ignored in FP64).  In
     // registers g is carried as deviations from w_i*T_c (see step_cell).
     const T tref_y = tref_row<T>(a.tref0, a.dtref, iy);
-    const T dref_up = tref_row<T>(a.tref0, a.dtref, dest_row(iy, 1)) - tref_y;   // written with c_y = +1
-    const T dref_dn = tref_row<T>(a.tref0, a.dtref, dest_row(iy, -1)) - tref_y;  // written with c_y = -1
+    const T dref_up = tref_row<T>(a.tref0, a.dtref, source_row(iy, 1)) - tref_y;   // streamed with c_y = +1
+    const T dref_dn = tref_row<T>(a.tref0, a.dtref, source_row(iy, -1)) - tref_y;  // streamed with c_y = -1
     // neighbour x - c lies inside for c = +1 (lo) / c = -1 (hi); x + c inside
     // for c = +1 is the hi test and for c = -1 the lo test
     const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -71,15 +72,17 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     T rho, ux, uy, uz, phi, dsum = T(0);
     {
         S fr[Q], gr[Q];
+        bool streamed[Q];  // the value came from the source cell x - c_i (inside the domain)
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             constexpr int cx = CX[i], cy = CY[i], cz = CZ[i], o = OPP[i];
+            const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
+            const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
+            const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+            streamed[i] = i != 0 && inx && iny && inz;
             if constexpr (MODE == MODE_AA_ODD && i != 0) {
-                const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
-                const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
-                const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
                 const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
-                const uint64_t off = (inx && iny && inz) ? at + o * ps - static_cast<uint64_t>(d) : at + i * ps;
+                const uint64_t off = streamed[i] ? at + o * ps - static_cast<uint64_t>(d) : at + i * ps;
                 fr[i] = ld<NTL && cx == 0>(&fs[off]);
                 gr[i] = ld<NTL && cx == 0>(&gs[off]);
             } else {
@@ -89,8 +92,18 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
         });
         static_for<Q>([&](auto I) {
             constexpr int i = I;
+            constexpr int cy = CY[i];
             f[i] = P::dec_f(fr[i]);
-            g[i] = EX ? P::template dec_g<i>(gr[i], tref_y) : P::dec_f(gr[i]);
+            if constexpr (EX) {
+                g[i] = P::template dec_g<i>(gr[i], tref_y);
+            } else {
+                // written by the source row when streamed, else by this cell
+                g[i] = P::dec_f(gr[i]);
+                if constexpr (cy != 0) {
+                    constexpr T w = static_cast<T>(weight(i));
+                    if (streamed[i]) g[i] = g[i] + w * (cy > 0 ? dref_up : dref_dn);
+                }
+            }
         });
     }
 
@@ -167,12 +180,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                         else gv = -gv + w2 * (twc + tw * rw);
                     }
                 }
-                // the slot's reader: cell x + c_j, or this cell for a wall
-                // value; 16-bit g is stored against the reader's row
-                if constexpr (!EX && cy != 0) {
-                    constexpr T w = static_cast<T>(weight(j));
-                    if (inside) gv = gv - w * (cy > 0 ? dref_up : dref_dn);
-                }
+                // 16-bit g is stored against this (the writer's) row
                 uint64_t dst = at + o * ps;
                 if constexpr (MODE == MODE_AA_ODD) {
                     const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);

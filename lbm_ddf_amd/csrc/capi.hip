@@ -117,7 +117,7 @@ lbm::StepArgs make_args(const lbm_ctx *c, const void *fa, const void *ga, void *
 
 lbm::CodecArgs codec_args(const lbm_ctx *c) {
     const lbm::StepArgs a = make_args(c, nullptr, nullptr, nullptr, nullptr);
-    return lbm::CodecArgs{a.lx, a.ly, a.rs, a.tref0, a.dtref};
+    return lbm::CodecArgs{a.lx, a.ly, a.lz, c->aa ? 1 : 0, a.rs, a.tref0, a.dtref};
 }
 
 hipError_t launch_step(const lbm_ctx *c, const lbm::StepArgs &a, int mode, bool reduce) {
@@ -501,7 +501,7 @@ extern "C" {
 int lbm_init_arrays(lbm_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
-    LBM_HIP(lbm::launch_init(a, c->prec, 0, 0, 0.0, c->stream));
+    LBM_HIP(lbm::launch_init(a, c->prec, 0, 0, 0.0, c->aa, c->stream));
     c->phase = PHASE_REF;
     c->aa_layout = lbm::AA_LAYOUT_N;
     c->nstep = 0;
@@ -511,7 +511,7 @@ int lbm_init_arrays(lbm_ctx *c) {
 int lbm_init_conductive(lbm_ctx *c, uint64_t seed, double amp) {
     if (int rc = check_ctx(c)) return rc;
     lbm::StepArgs a = make_args(c, nullptr, nullptr, c->f[c->cf], c->g[c->cg]);
-    LBM_HIP(lbm::launch_init(a, c->prec, 1, seed, amp, c->stream));
+    LBM_HIP(lbm::launch_init(a, c->prec, 1, seed, amp, c->aa, c->stream));
     c->phase = PHASE_REF;
     c->aa_layout = lbm::AA_LAYOUT_N;
     c->nstep = 0;

@@ -82,8 +82,10 @@ hipError_t launch_aa_exchange(int precision, void *f, void *g, uint64_t ps, uint
 hipError_t launch_aa_flip(const StepArgs &a, bool scalar, hipStream_t s);
 
 // Reference-state stage kernels (FP64).
+// writer_rows: 16-bit g of the in-place pattern, encoded against the row of
+// the cell that wrote each slot (tref.hpp writer_row).
 hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
-                       hipStream_t s);
+                       bool writer_rows, hipStream_t s);
 hipError_t launch_forcing(const StepArgs &a, hipStream_t s);
 hipError_t launch_collide_f(const StepArgs &a, hipStream_t s);
 hipError_t launch_collide_g(const StepArgs &a, hipStream_t s);
@@ -106,7 +108,8 @@ hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s
 
 // Population plane conversion between the storage format and FP64.
 struct CodecArgs {
-    int32_t lx, ly;
+    int32_t lx, ly, lz;
+    int32_t writer_rows;  // 16-bit in-place storage: g slots encoded against their writer's row
     uint64_t rs;          // row stride of the population layout
     double tref0, dtref;  // row reference temperature of the g storage
 };

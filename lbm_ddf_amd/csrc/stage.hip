@@ -42,14 +42,14 @@ __device__ __forceinline__ double perturbation(uint64_t seed, uint64_t idx, doub
 
 // init_arrays, src/initial.cu:208-242 (conductive: phi from the linear
 // profile of Parallel/initial.cu:89 plus the seeded perturbation).
+// writer_rows: 16-bit in-place storage, whose reference state encodes each
+// slot against the row of the cell that streamed it in (tref.hpp).
 template <class P>
-__global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double amp) {
+__global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double amp, int writer_rows) {
     const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= a.n) return;
     int ix, iy, iz;
     cell_coords(idx, a.lx, a.ly, ix, iy, iz);
-    (void)ix;
-    (void)iz;
     double phi = 0.5;
     if (conductive) {
         const double t = a.t_hot - (a.t_hot - a.t_cold) * iy / (a.ly - 1.0);
@@ -67,12 +67,13 @@ __global__ void init_kernel(StepArgs a, int conductive, uint64_t seed, double am
     const double fy = a.fy[idx];
     auto *f = static_cast<typename P::S *>(a.fb);
     auto *g = static_cast<typename P::S *>(a.gb);
-    const typename P::T tref = tref_row<typename P::T>(a.tref0, a.dtref, iy);
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         const double feq = weight(i) * rho * (1.0 - 1.5 * usq);
         const double fv = feq - 0.5 * fy * CY[i] / (1.0 / 3.0) * feq;
         const double geq = weight(i) * phi * (1.0 - 1.5 * usq);
+        const int row = writer_rows ? writer_row(ix, iy, iz, a.lx, a.ly, a.lz, CX[i], CY[i], CZ[i]) : iy;
+        const typename P::T tref = tref_row<typename P::T>(a.tref0, a.dtref, row);
         f[at + i * ps] = P::enc_f(static_cast<typename P::T>(fv));
         g[at + i * ps] = P::template enc_g<i>(static_cast<typename P::T>(geq), tref);
     });
@@ -302,11 +303,18 @@ __global__ __launch_bounds__(kProfileThreads) void profile_kernel(StepArgs a, do
     }
 }
 
-// Row reference temperature of the 16-bit g storage at flat cell index i:
-// the binary32 value the step kernels encode and decode with, in binary64.
+// Reference temperature of the 16-bit g storage of population IP at flat
+// cell index i: the binary32 value the step kernels encode and decode that
+// slot with (its own row; with writer_rows the row of the cell that wrote
+// the slot), in binary64.
+template <int IP>
 __device__ __forceinline__ double codec_tref(uint64_t i, const CodecArgs &c) {
-    const uint64_t iy = (i / static_cast<uint64_t>(c.lx)) % static_cast<uint64_t>(c.ly);
-    return static_cast<double>(tref_row<float>(c.tref0, c.dtref, static_cast<int>(iy)));
+    const uint64_t r = i / static_cast<uint64_t>(c.lx);
+    const int ix = static_cast<int>(i - r * static_cast<uint64_t>(c.lx));
+    const int iy = static_cast<int>(r % static_cast<uint64_t>(c.ly));
+    const int iz = static_cast<int>(r / static_cast<uint64_t>(c.ly));
+    const int row = c.writer_rows ? writer_row(ix, iy, iz, c.lx, c.ly, c.lz, CX[IP], CY[IP], CZ[IP]) : iy;
+    return static_cast<double>(tref_row<float>(c.tref0, c.dtref, row));
 }
 
 // Host copies convert in binary64: a decoded value is the stored value
@@ -333,7 +341,7 @@ __global__ void decode_kernel(const void *src, double *dst, uint64_t n, CodecArg
     if (i >= n) return;
     const auto s = static_cast<const typename P::S *>(src)[slot_of(i, c.lx, c.rs)];
     if constexpr (WHICH == 0 || P::exact) dst[i] = stored_deviation<P>(s);
-    else dst[i] = stored_deviation<P>(s) + weight(IP) * codec_tref(i, c);
+    else dst[i] = stored_deviation<P>(s) + weight(IP) * codec_tref<IP>(i, c);
 }
 
 template <class P, int WHICH, int IP>
@@ -343,7 +351,7 @@ __global__ void encode_kernel(const double *src, void *dst, uint64_t n, CodecArg
     auto *d = static_cast<typename P::S *>(dst);
     const uint64_t at = slot_of(i, c.lx, c.rs);
     if constexpr (WHICH == 0 || P::exact) d[at] = store_deviation<P>(src[i]);
-    else d[at] = store_deviation<P>(src[i] - weight(IP) * codec_tref(i, c));
+    else d[at] = store_deviation<P>(src[i] - weight(IP) * codec_tref<IP>(i, c));
 }
 
 template <class P>
@@ -385,16 +393,17 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src
 }  // namespace
 
 hipError_t launch_init(const StepArgs &a, int precision, int conductive, uint64_t seed, double amp,
-                       hipStream_t s) {
+                       bool writer_rows, hipStream_t s) {
+    const int wr = writer_rows ? 1 : 0;
     switch (precision) {
         case LBM_PREC_FP64:
-            hipLaunchKernelGGL(init_kernel<PolicyF64>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp);
+            hipLaunchKernelGGL(init_kernel<PolicyF64>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp, wr);
             break;
         case LBM_PREC_FP16:
-            hipLaunchKernelGGL(init_kernel<PolicyF16>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp);
+            hipLaunchKernelGGL(init_kernel<PolicyF16>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp, wr);
             break;
         case LBM_PREC_BF16:
-            hipLaunchKernelGGL(init_kernel<PolicyBF16>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp);
+            hipLaunchKernelGGL(init_kernel<PolicyBF16>, flat_grid(a.n), dim3(kFlatBlock), 0, s, a, conductive, seed, amp, wr);
             break;
         default:
             return hipErrorInvalidValue;

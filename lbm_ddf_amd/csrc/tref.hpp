@@ -5,11 +5,11 @@
 // walls half a cell outside the end rows.  A stored value is often decoded
 // by a cell of another row than the one that encoded it: in the pull scheme
 // a cell decodes what its source row stored, and in the in-place pattern a
-// cell encodes for the row that will read the slot.  Both sides must use
+// cell writes into its neighbours' slots.  Both sides must use
 // bit-identical temperatures for a row, so every encode and decode of every
 // kernel, the initial state and the host-copy codec take T_ref from
-// tref_row() with the row index of the cell that owns the stored value
-// (source_row / dest_row below).  Recomputing a neighbour row's value from
+// tref_row() with the row index of the cell that wrote the stored value
+// (source_row and writer_row below).  Recomputing a neighbour row's value from
 // the own row's binary32 value (T_ref(iy) + c_y * dtref in binary32) would
 // disagree by an ulp for about a quarter of the rows unless LY is a power of
 // two, and inject a fixed per-row heat bias.
@@ -36,5 +36,16 @@ constexpr int source_row(int iy, int cy) { return iy - cy; }
 // Row of the cell that will read a slot written along a direction with y
 // component cy (the destination cell x + c).
 constexpr int dest_row(int iy, int cy) { return iy + cy; }
+
+// In-place pattern, layout N (the reference's post-streaming layout): the row
+// of the cell that wrote slot (i, x) of cell x = (ix, iy, iz), i of velocity
+// c = (cx, cy, cz).  The slot holds f*_i(x - c) streamed from the source cell
+// when that lies inside the domain, else the wall value that x itself wrote.
+// 16-bit in-place storage encodes every value against its writer's row, so
+// the stored deviations are as small as the A/B scheme's.
+constexpr int writer_row(int ix, int iy, int iz, int lx, int ly, int lz, int cx, int cy, int cz) {
+    const bool inside = ix - cx >= 0 && ix - cx < lx && iy - cy >= 0 && iy - cy < ly && iz - cz >= 0 && iz - cz < lz;
+    return inside ? iy - cy : iy;
+}
 
 }  // namespace lbm

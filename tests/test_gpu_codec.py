@@ -42,10 +42,28 @@ def t_ref(shape):
     return tr[None, None, :, None]  # broadcast over (27, z, y, x)
 
 
-def expected(prec, which, values, shape):
+def t_ref_writer(shape):
+    """In-place storage: T_ref of the row of the cell that wrote each slot
+    (i, z, y, x) of the reference state -- the source cell x - c_i when it
+    lies inside the domain, else the cell itself (lbm_ddf_amd/csrc/tref.hpp)."""
+    lz, ly, lx = shape
+    cx, cy, cz, _ = oracle.tables()
+    dtref = (WALLS["t_hot"] - WALLS["t_cold"]) / ly
+    tref0 = WALLS["t_hot"] - 0.5 * dtref
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    out = np.empty((27,) + shape)
+    for i in range(27):
+        inside = ((x - cx[i] >= 0) & (x - cx[i] < lx) & (y - cy[i] >= 0) & (y - cy[i] < ly)
+                  & (z - cz[i] >= 0) & (z - cz[i] < lz))
+        row = np.where(inside, y - cy[i], y)
+        out[i] = (tref0 - row * dtref).astype(np.float32).astype(np.float64)
+    return out
+
+
+def expected(prec, which, values, shape, writer_rows=False):
     """numpy emulation of encode-then-decode of the host copies."""
     w = W[:, None, None, None]
-    tr = t_ref(shape)
+    tr = t_ref_writer(shape) if writer_rows else t_ref(shape)
     d = values - w * tr if which == "g" else values
     if prec == "fp16":
         h = (d * 32768.0).astype(np.float32).astype(np.float16)
@@ -142,3 +160,23 @@ def test_16bit_step_is_the_fp64_step_within_its_roundings(prec, rel, nsteps, rou
         assert np.all(err <= bound), f"{prec} {which}: error {err[worst]:.3e} > bound {bound[worst]:.3e} at {worst}"
     # phi is the FP32 sum of the decoded g: the same roundings, summed
     assert np.max(np.abs(c.get_field("phi") - o["phi"])) < 27 * roundings * rel * 0.02 + 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_codec_roundtrip_in_place_writer_rows(prec):
+    """In place, the reference state encodes g of each slot against the row of
+    the cell that streamed it in (its own row for a wall value): set_pops then
+    get_pops equals the numpy emulation of that rule bit for bit, and writing
+    back what was read changes nothing."""
+    shape = (20, 12, 10)
+    zyx = shape[::-1]
+    c = Cavity(*shape, precision=prec, pattern="aa", **WALLS)
+    c.init_conductive(1, 1.0e-3)
+    rng = np.random.default_rng(5)
+    vals = sample(zyx, "g", rng)
+    c.set_pops("g", vals)
+    got = c.get_pops("g")
+    np.testing.assert_array_equal(got, expected(prec, "g", vals, zyx, writer_rows=True))
+    c.set_pops("g", got)
+    np.testing.assert_array_equal(c.get_pops("g"), got)
+    c.close()
