@@ -525,6 +525,8 @@ def main() -> int:
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per candidate for the launch-shape autotuner (0: defaults)")
+    ap.add_argument("--tuning", type=int, nargs=2, default=None, metavar=("BLOCK_X", "FLAGS"),
+                    help="single workload: fixed launch shape instead of the autotuner")
     ap.add_argument("--workload", default="single",
                     choices=["single", "ensemble", "serve", "precision", "precision-report", "precision-seeds",
                              "precision-seeds-report", "config1"],
@@ -690,7 +692,10 @@ def main() -> int:
     t = time.perf_counter()
     tuned = None
     warm = args.warmup
-    if args.autotune > 0:
+    if args.tuning:
+        c.set_tuning(*args.tuning)
+        log(f"fixed launch shape: block_x={args.tuning[0]} flags={args.tuning[1]}")
+    elif args.autotune > 0:
         tuned = c.autotune(args.autotune)
         warm = max(0, warm - tuned["steps_taken"])
         log(f"autotune: block_x={tuned['block_x']} flags={tuned['flags']} "

@@ -115,7 +115,11 @@ const char *lbm_last_error(void);
 /* src/globals.cu:11-23 (host-side parameter derivation). */
 int lbm_derive(const lbm_params *p, lbm_derived *out);
 
-/* Device bytes a context with these parameters allocates. */
+/* Device bytes a context with these parameters allocates at creation:
+ * populations, fields, side values, diagnostic partials and records, each
+ * allocation rounded up to 4 KiB.  The first host copy of populations
+ * (lbm_get_pops, lbm_set_pops, checkpoints) adds a staging plane of LX*LY*LZ
+ * doubles. */
 int lbm_bytes_required(const lbm_params *p, uint64_t *bytes);
 
 /* initialize(), src/initial.cu:3-67 (allocation) and :86-99 (constants).
