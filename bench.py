@@ -16,7 +16,8 @@ Other workloads (--workload): ensemble (64 cavities in one launch per step
 against the same cavities one after another), serve (Poisson arrivals
 through the request loop), precision / precision-report (16-bit storage
 against FP64 over a long run), config1 (the 64^3 1000-step parity case with
-GPU and oracle both timed).
+GPU and oracle both timed), patterns (copy kernels with each pass's access
+pattern and no arithmetic, the bandwidth bound of each kernel).
 """
 import argparse
 import json
@@ -51,6 +52,66 @@ def traffic_from_profiles(workload_key: str):
             except ValueError:
                 return None
     return None
+
+
+def pattern_reference(cavity, pattern: str, reps: int = 10) -> dict:
+    """Time of one pass of the copy kernel that moves the fused pass's
+    population bytes along the same addresses with no arithmetic, run on the
+    cavity's own population buffers and launch shape (lbm_pattern_time): the
+    bandwidth the access pattern itself allows on the same physical pages.
+    A/B: the pull gather, both directions; in place: the even and odd passes.
+    Overwrites the cavity's populations (called after its last use)."""
+    ms0, ms1 = cavity.pattern_time(reps)
+    out = {"ms": round((ms0 + ms1) / 2, 4), "reps": reps}
+    if pattern == "aa":
+        out.update(access="in-place even and odd passes (mean)", even_ms=round(ms0, 4), odd_ms=round(ms1, 4))
+    else:
+        out.update(access="pull gather between the two buffers (mean of both directions)")
+    return out
+
+
+def run_patterns_workload(args) -> dict:
+    """Every access pattern at both element sizes on a --size cube, all on
+    the same two device buffers (in-place patterns on the first), timed with
+    HIP events over --steps passes after two untimed ones; launch --tuning,
+    else block 256 with flags 7.  GB/s counts the population bytes
+    (2 * 54 * elem bytes per cell)."""
+    import torch
+
+    from lbm_ddf_amd import capi
+
+    L = args.size
+    lz = args.lz if args.lz > 0 else L
+    n = L * L * lz
+    bx, fl = args.tuning if args.tuning else (256, 7)
+    reps = max(1, args.steps)
+    a = torch.zeros(54 * n * 8, dtype=torch.uint8, device="cuda")
+    b = torch.zeros_like(a)
+    stream = torch.cuda.current_stream()
+    rows = []
+    for elem in (8, 2):
+        for name in capi.ACCESS_PATTERNS:
+            inplace = name.startswith("inplace")
+
+            def one(r):
+                src, dst = (a, a) if inplace else ((a, b) if r % 2 == 0 else (b, a))
+                capi.pattern_copy(name, elem, (L, L, lz), src.data_ptr(), dst.data_ptr(), bx, fl, stream.cuda_stream)
+
+            for r in range(2):
+                one(r)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for r in range(reps):
+                one(r)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbps = 2 * 54 * elem * n / (ms * 1e-3) / 1e9
+            rows.append({"pattern": name, "elem_bytes": elem, "ms": round(ms, 4), "gbps": round(gbps, 1),
+                         "frac_of_peak": round(gbps / HBM_PEAK_GBS, 4)})
+            log(f"pattern {name:13s} elem {elem}: {ms:8.3f} ms  {gbps:7.1f} GB/s")
+    del a, b
+    return {"shape": [L, L, lz], "cells": n, "launch": {"block_x": bx, "flags": fl}, "reps": reps, "patterns": rows}
 
 
 def cpu_topology():
@@ -529,8 +590,9 @@ def main() -> int:
                     help="single workload: fixed launch shape instead of the autotuner")
     ap.add_argument("--workload", default="single",
                     choices=["single", "ensemble", "serve", "precision", "precision-report", "precision-seeds",
-                             "precision-seeds-report", "config1"],
-                    help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size")
+                             "precision-seeds-report", "config1", "patterns"],
+                    help="single: one 512^3 cavity (default); ensemble: 64 cavities of uneven size; "
+                         "patterns: the access-pattern copies (--steps passes each)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--config1-steps", type=int, default=1000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
@@ -634,6 +696,20 @@ def main() -> int:
         }), flush=True)
         return 0
 
+    if args.workload == "patterns":
+        res = run_patterns_workload(args)
+        pull = next(r for r in res["patterns"] if r["pattern"] == "pull" and r["elem_bytes"] == 8)
+        print(json.dumps({
+            "metric": "GB/s of the copy kernels with the step passes' access patterns (no arithmetic)",
+            "value": pull["gbps"], "unit": "GB/s", "n_gpus": 1, "steps": res["reps"], "warmup": 2,
+            "ms_per_step": pull["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 and 16-bit populations, copied", "data": "synthetic: zero-filled buffers",
+            "config": {"workload": f"54 populations per cell, {res['shape'][0]}x{res['shape'][1]}x{res['shape'][2]}"
+                                   " row-blocked; value: the pull pattern at 8 bytes", "launch": res["launch"]},
+            "patterns": res["patterns"],
+        }), flush=True)
+        return 0
+
     if args.workload == "serve":
         res = run_serve_workload(args)
         st = res["stats"]
@@ -715,6 +791,7 @@ def main() -> int:
     d = c.reduce()
     if d["nonfinite"] != 0:
         log("WARNING: non-finite values in the final state")
+    pattern_ref = pattern_reference(c, args.pattern)
     c.close()
     # the second roofline reference: a plain device copy (4 GiB, read + write)
     from lbm_ddf_amd import capi as _capi
@@ -765,6 +842,8 @@ def main() -> int:
             "copy_gbps": round(copy_gbps, 1),
             "frac_of_copy": round(achieved / copy_gbps, 4),
             "traffic_rate_of_copy": round(traffic / avg_launch_s / 1e9 / copy_gbps, 4) if traffic else None,
+            "pattern_copy": pattern_ref,
+            "frac_of_pattern": round(pattern_ref["ms"] / (avg_launch_s * 1e3), 4),
         },
         "physics": {"nu_bulk": d["nu_bulk"], "nu_top": d["nu_top"], "nu_bottom": d["nu_bottom"],
                     "mass": d["mass"], "kinetic_energy": d["kinetic_energy"],
@@ -803,6 +882,7 @@ def fp16_leg(args, fp64_mlups: float, copy_gbps: float) -> dict:
     wall = time.perf_counter() - t0
     d = c.reduce()
     bx, fl = c.tuning
+    pattern_ref = pattern_reference(c, args.pattern)
     c.close()
     mlups = n * steps / wall / 1e6
     achieved = BYTES_PER_CELL["fp16"] * n / (kernel_ms / 1e3 / steps) / 1e9
@@ -813,7 +893,9 @@ def fp16_leg(args, fp64_mlups: float, copy_gbps: float) -> dict:
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "frac_of_copy": round(achieved / copy_gbps, 4),
                          "traffic": traffic_from_profiles(f"fused_fp16_{shape}_bx{bx}_f{fl}")
                          or traffic_from_profiles(f"fused_fp16_{shape}"),
-                         "algorithmic_bytes_per_launch": BYTES_PER_CELL["fp16"] * n},
+                         "algorithmic_bytes_per_launch": BYTES_PER_CELL["fp16"] * n,
+                         "pattern_copy": pattern_ref,
+                         "frac_of_pattern": round(pattern_ref["ms"] / (kernel_ms / steps), 4)},
             "launch": {"block_x": bx, "flags": fl, "autotuned": tuned is not None},
             "nonfinite": d["nonfinite"],
             "note": "same workload with populations stored as FP16 deviations, FP32 arithmetic on deviations; "

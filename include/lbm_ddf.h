@@ -218,6 +218,41 @@ int lbm_autotune(lbm_ctx *ctx, int32_t steps, int64_t *steps_taken, double *best
  * reference for the roofline fraction. */
 int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps);
 
+/* Access-pattern copies (no reference counterpart): copy kernels that move
+ * the 54 populations of every cell (elem_bytes 8: FP64 storage; 2: 16-bit
+ * storage) in the row-blocked layout of a single cavity -- stream s (f_i at
+ * s = i, g_i at s = 27 + i) of cell (ix, iy, iz) at element
+ * ((iz*ly + iy)*54 + s)*lx + ix -- along the addresses of one step pass, with
+ * no arithmetic: the achievable-bandwidth reference of that pass.  Buffers
+ * hold 54*lx*ly*lz elements; the in-place patterns take dst == src, the
+ * others dst != src.  block_x: 128, 256 or 512; flags: 3 or 7
+ * (lbm_set_tuning's bits: non-temporal stores and non-temporal unshifted
+ * loads always, bit 2 the XCD-aware block order).  lbm_pattern_copy enqueues
+ * one pass on `stream` (a hipStream_t; NULL: the default stream).
+ * lbm_pattern_bandwidth allocates its own buffers, runs `reps` passes after
+ * two untimed ones and returns the mean time of a pass in milliseconds. */
+#define LBM_ACCESS_COPY 0         /* own slots -> own slots of a second buffer */
+#define LBM_ACCESS_PULL 1         /* the A/B pass: gather from x - c_i, wall slots bounced */
+#define LBM_ACCESS_PUSH 2         /* own slots -> scatter to x + c_j of a second buffer */
+#define LBM_ACCESS_INPLACE_EVEN 3 /* the in-place even pass: own slots, one buffer */
+#define LBM_ACCESS_INPLACE_ODD 4  /* the in-place odd pass: neighbours' slots, one buffer */
+int lbm_pattern_copy(int32_t pattern, int32_t elem_bytes, int32_t lx, int32_t ly, int32_t lz, int32_t block_x,
+                     int32_t flags, const void *src, void *dst, void *stream);
+int lbm_pattern_bandwidth(int32_t pattern, int32_t elem_bytes, int32_t lx, int32_t ly, int32_t lz, int32_t block_x,
+                          int32_t flags, int32_t reps, double *ms_per_pass);
+/* The access-pattern copy of the context's own fused pass, timed on the
+ * context's own population buffers, stream and launch shape (block_x outside
+ * 128/256/512 becomes the nearest of them; flags 7 when the context's bit 2
+ * is set, else 3), so that the fused kernel and its pattern copy read and
+ * write the same physical pages.  ms_per_pass[0]: A/B, the pull gather from
+ * buffer 0 into buffer 1; in place, the even pass.  ms_per_pass[1]: A/B, the
+ * pull back from buffer 1 into buffer 0; in place, the odd pass.  Each is the
+ * mean over `reps` passes after two untimed ones.  The copies overwrite the
+ * populations: afterwards the context has no state (lbm_init_* before
+ * stepping again).  Single-cavity contexts only (LBM_E_STATE for an ensemble
+ * member). */
+int lbm_pattern_time(lbm_ctx *ctx, int32_t reps, double *ms_per_pass);
+
 /* ---- Checkpoints in the reference's binary formats ------------------------
  * lbm_output_flow: output_flow (src/output.cu:67-146): rho', u_x, u_y, u_z,
  *   phi, each LX*LY*LZ doubles, x fastest, native byte order.

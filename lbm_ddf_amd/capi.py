@@ -100,6 +100,10 @@ SIGNATURES = {
     "lbm_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "lbm_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _D]),
     "lbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, _D]),
+    # pattern, elem_bytes, lx, ly, lz, block_x, flags, src, dst (device pointers as integers), stream
+    "lbm_pattern_copy": (ctypes.c_int, [ctypes.c_int32] * 7 + [ctypes.c_void_p] * 3),
+    "lbm_pattern_bandwidth": (ctypes.c_int, [ctypes.c_int32] * 8 + [_D]),
+    "lbm_pattern_time": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
     "lbm_output_flow": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "lbm_output_fg": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "lbm_restart": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]),
@@ -177,6 +181,29 @@ def copy_bandwidth(nbytes: int = 4 << 30, reps: int = 10) -> float:
     """GB/s (read + write) of a plain device copy kernel."""
     v = ctypes.c_double()
     _check(load().lbm_copy_bandwidth(ctypes.c_uint64(int(nbytes)), int(reps), ctypes.byref(v)))
+    return v.value
+
+
+# Access patterns of lbm_pattern_copy (include/lbm_ddf.h LBM_ACCESS_*).
+ACCESS_PATTERNS = {"copy": 0, "pull": 1, "push": 2, "inplace_even": 3, "inplace_odd": 4}
+
+
+def pattern_copy(pattern: str, elem_bytes: int, shape: tuple, src_ptr: int, dst_ptr: int, block_x: int = 256,
+                 flags: int = 7, stream: int = 0) -> None:
+    """Enqueue one access-pattern copy over device buffers of 54*lx*ly*lz elements."""
+    lx, ly, lz = shape
+    _check(load().lbm_pattern_copy(ACCESS_PATTERNS[pattern], int(elem_bytes), int(lx), int(ly), int(lz), int(block_x),
+                                   int(flags), ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr),
+                                   ctypes.c_void_p(stream or None)))
+
+
+def pattern_bandwidth(pattern: str, elem_bytes: int, shape: tuple, block_x: int = 256, flags: int = 7,
+                      reps: int = 10) -> float:
+    """Mean milliseconds of one access-pattern copy pass (own buffers)."""
+    lx, ly, lz = shape
+    v = ctypes.c_double()
+    _check(load().lbm_pattern_bandwidth(ACCESS_PATTERNS[pattern], int(elem_bytes), int(lx), int(ly), int(lz),
+                                        int(block_x), int(flags), int(reps), ctypes.byref(v)))
     return v.value
 
 
@@ -266,6 +293,14 @@ class Cavity:
         ms = ctypes.c_double()
         _check(self.lib.lbm_step_timed(self.h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def pattern_time(self, reps: int = 10) -> tuple:
+        """(ms, ms) of the access-pattern copy of this context's fused pass
+        on its own buffers (A/B: pull 0->1 and 1->0; in place: even, odd).
+        Overwrites the populations: initialise again before stepping."""
+        ms = (ctypes.c_double * 2)()
+        _check(self.lib.lbm_pattern_time(self.h, int(reps), ms))
+        return ms[0], ms[1]
 
     def run_diag(self, n: int, every: int) -> list:
         cap = n // every + 1

@@ -775,6 +775,33 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int64_t *steps_taken, double *best_m
     return LBM_OK;
 }
 
+int lbm_pattern_time(lbm_ctx *c, int32_t reps, double *ms_per_pass) {
+    if (int rc = check_own(c, "lbm_pattern_time")) return rc;
+    if (!ms_per_pass || reps < 1) return fail(LBM_E_INVALID, "need reps >= 1 and an output array of two");
+    const int lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
+    if (c->ps != static_cast<uint64_t>(lx) || c->rs != 54ull * static_cast<uint64_t>(lx))
+        return fail(LBM_E_UNSUPPORTED, "pattern timing needs the row-blocked single-cavity layout");
+    const int elem = c->prec == LBM_PREC_FP64 ? 8 : 2;
+    const int bx = c->bx <= 128 ? 128 : (c->bx <= 256 ? 256 : 512);
+    const int fl = (c->flags & 4) ? 7 : 3;  // bit 2: the XCD-aware block order
+    if (lbm::step_grid_blocks(lx, ly, lz, bx) > 0x7fffffffLL) return fail(LBM_E_INVALID, "grid too large");
+    c->phase = PHASE_EMPTY;  // the copies overwrite the populations
+    for (int par = 0; par < 2; ++par) {
+        const int pat = c->aa ? (par == 0 ? LBM_ACCESS_INPLACE_EVEN : LBM_ACCESS_INPLACE_ODD) : LBM_ACCESS_PULL;
+        const void *src = c->aa ? c->buf[0] : c->buf[par];
+        void *dst = c->aa ? c->buf[0] : c->buf[1 - par];
+        for (int r = 0; r < 2; ++r) LBM_HIP(lbm::launch_pattern(pat, elem, src, dst, lx, ly, lz, bx, fl, c->stream));
+        LBM_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; ++r) LBM_HIP(lbm::launch_pattern(pat, elem, src, dst, lx, ly, lz, bx, fl, c->stream));
+        LBM_HIP(hipEventRecord(c->ev1, c->stream));
+        LBM_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        LBM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms_per_pass[par] = static_cast<double>(ms) / reps;
+    }
+    return LBM_OK;
+}
+
 int lbm_copy_bandwidth(uint64_t bytes, int32_t reps, double *gbps) {
     if (!gbps || reps < 1 || bytes < 4096) return fail(LBM_E_INVALID, "bad arguments");
     bytes &= ~static_cast<uint64_t>(4095);

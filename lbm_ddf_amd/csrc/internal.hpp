@@ -105,6 +105,10 @@ hipError_t launch_profile(const StepArgs &a, double *out, hipStream_t s);
 
 // Plain device copy of bytes (multiple of 16) for the bandwidth reference.
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+// One access-pattern copy (LBM_ACCESS_*) over 54 row-blocked streams of
+// elem_bytes 8 or 2; block_x 128/256/512, flags 3 or 7 (pattern.hip).
+hipError_t launch_pattern(int pattern, int elem_bytes, const void *src, void *dst, int lx, int ly, int lz, int bx,
+                          int flags, hipStream_t s);
 
 // Population plane conversion between the storage format and FP64.
 struct CodecArgs {

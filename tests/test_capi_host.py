@@ -166,3 +166,40 @@ def test_cell_count_limit():
     assert b"2^32" in lib.lbm_last_error()
     p = capi.make_params(2048, 2048, 1023)
     assert lib.lbm_bytes_required(ctypes.byref(p), ctypes.byref(b)) == 0
+
+
+@pytest.mark.parametrize("args,code,msg", [
+    ((9, 8, 8, 4, 4, 256, 7), -1, b"access pattern"),
+    ((1, 4, 8, 4, 4, 256, 7), -1, b"elem_bytes"),
+    ((1, 8, 0, 4, 4, 256, 7), -1, b"lx, ly, lz"),
+    ((1, 8, 8, 4, 4, 64, 7), -5, b"block_x"),
+    ((1, 8, 8, 4, 4, 256, 5), -5, b"flags"),
+])
+def test_pattern_copy_rejects_bad_arguments_without_touching_the_gpu(args, code, msg):
+    lib = capi.load()
+    buf = ctypes.create_string_buffer(64)
+    src, dst = ctypes.c_void_p(ctypes.addressof(buf)), ctypes.c_void_p(ctypes.addressof(buf) + 32)
+    assert lib.lbm_pattern_copy(*args, src, dst, None) == code
+    assert msg in lib.lbm_last_error()
+    v = ctypes.c_double(-1.0)
+    assert lib.lbm_pattern_bandwidth(*args, 3, ctypes.byref(v)) == code
+    assert v.value == -1.0
+
+
+def test_pattern_copy_checks_buffers_before_launching():
+    """In-place patterns take one buffer, the others two distinct ones; NULL
+    buffers and reps < 1 are refused before any device call."""
+    lib = capi.load()
+    buf = ctypes.create_string_buffer(64)
+    a, b = ctypes.c_void_p(ctypes.addressof(buf)), ctypes.c_void_p(ctypes.addressof(buf) + 32)
+    shape = (8, 4, 4, 256, 7)
+    for pat in (capi.ACCESS_PATTERNS["inplace_even"], capi.ACCESS_PATTERNS["inplace_odd"]):
+        assert lib.lbm_pattern_copy(pat, 8, *shape, a, b, None) == -1
+        assert b"dst == src" in lib.lbm_last_error()
+    for pat in (capi.ACCESS_PATTERNS["copy"], capi.ACCESS_PATTERNS["pull"], capi.ACCESS_PATTERNS["push"]):
+        assert lib.lbm_pattern_copy(pat, 2, *shape, a, a, None) == -1
+        assert b"dst must differ" in lib.lbm_last_error()
+    assert lib.lbm_pattern_copy(1, 8, *shape, None, b, None) == -1
+    v = ctypes.c_double()
+    assert lib.lbm_pattern_bandwidth(1, 8, *shape, 0, ctypes.byref(v)) == -1
+    assert lib.lbm_pattern_bandwidth(1, 8, *shape, 3, None) == -1
