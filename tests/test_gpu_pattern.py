@@ -118,11 +118,20 @@ def test_pattern_time_on_a_cavity_leaves_it_uninitialised(precision, pattern):
         assert 0.0 < ms0 < 1000.0 and 0.0 < ms1 < 1000.0
         with pytest.raises(RuntimeError, match="no state"):
             c.step(1)
-        c.init_conductive(1, 1e-3)  # a fresh start works again
-        c.step(2)
-        assert c.reduce()["nonfinite"] == 0
+        c.init_conductive(1, 1e-3)  # a fresh start works again ...
+        c.step(4)
+        got = c.reduce()
     finally:
         c.close()
+    fresh = Cavity(70, 12, 8, rayleigh=1e5, prandtl=0.71, t_hot=0.5, t_cold=-0.5, precision=precision,
+                   pattern=pattern)
+    try:
+        fresh.init_conductive(1, 1e-3)
+        fresh.step(4)
+        want = fresh.reduce()
+    finally:
+        fresh.close()
+    assert got == want  # ... and steps exactly like a new cavity
 
 
 def test_pattern_time_refuses_an_ensemble_member():
