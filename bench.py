@@ -451,7 +451,8 @@ def run_precision_seeds_workload(args) -> dict:
 
     L, prec = args.size, args.precision
     e = Ensemble(prec)
-    slots = [e.add(L, L, L, seed=k + 1, amp=1.0e-3, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
+    first = args.seed_offset + 1
+    slots = [e.add(L, L, L, seed=first + k, amp=1.0e-3, rayleigh=args.ra, prandtl=0.71, t_hot=0.5, t_cold=-0.5)
              for k in range(args.seeds)]
     e.set_tuning(*args.seed_tuning)
     series = {s: [] for s in slots}
@@ -476,19 +477,20 @@ def run_precision_seeds_workload(args) -> dict:
     for s in slots:
         a = np.array(series[s])
         sel = a[a[:, 0] > args.avg_from]
-        members.append({"seed": s + 1, "nu_bulk": float(sel[:, 1].mean()), "nu_top": float(sel[:, 2].mean()),
+        members.append({"seed": first + s, "nu_bulk": float(sel[:, 1].mean()), "nu_top": float(sel[:, 2].mean()),
                         "nu_bottom": float(sel[:, 3].mean()), "kinetic_energy": float(sel[:, 4].mean()),
                         "mass_drift_per_cell": float((a[-1, 5] - a[0, 5]) / L ** 3),
                         "nonfinite": float(a[:, 6].max()), "samples": int(len(sel))})
-    return {"precision": prec, "size": L, "ra": args.ra, "seeds": args.seeds, "steps": args.steps,
+    return {"precision": prec, "size": L, "ra": args.ra, "seeds": args.seeds, "first_seed": first, "steps": args.steps,
             "avg_from": args.avg_from, "sample": args.sample,
             "mlups": args.seeds * L ** 3 * args.steps / wall / 1e6, "members": members,
-            "series": {str(s + 1): series[s] for s in slots}}
+            "series": {str(first + s): series[s] for s in slots}}
 
 
 def precision_seeds_report(size: int, ra: float, out_dir: str) -> dict:
     """Seed-averaged time-mean Nu of each precision against FP64 (files the
-    precision-seeds workload wrote), with 95% confidence intervals of the
+    precision-seeds workload wrote, several runs of one precision with
+    different --seed-offset pooled), with 95% confidence intervals of the
     differences from the seed-to-seed spread (Welch)."""
     import numpy as np
     from scipy.stats import t as student_t
@@ -496,15 +498,25 @@ def precision_seeds_report(size: int, ra: float, out_dir: str) -> dict:
     d = Path(out_dir)
     runs = {}
     for prec in ("fp64", "fp16", "bf16"):
-        f = d / f"precision_seeds_{prec}_{size}_{ra:g}.json"
-        if f.exists():
-            runs[prec] = json.loads(f.read_text())
+        parts = [json.loads(f.read_text()) for f in sorted(d.glob(f"precision_seeds_{prec}_{size}_{ra:g}*.json"))]
+        if not parts:
+            continue
+        seeds = [m["seed"] for r in parts for m in r["members"]]
+        if len(set(seeds)) != len(seeds):
+            raise ValueError(f"{prec}: the same seed appears in two runs of {d}")
+        if len({(r["steps"], r["avg_from"], r["sample"]) for r in parts}) != 1:
+            raise ValueError(f"{prec}: runs with different step windows in {d}")
+        walls = sum(r["seeds"] * size ** 3 * r["steps"] / r["mlups"] / 1e6 for r in parts)
+        runs[prec] = {"seeds": len(seeds), "steps": parts[0]["steps"], "avg_from": parts[0]["avg_from"],
+                      "mlups": len(seeds) * size ** 3 * parts[0]["steps"] / walls / 1e6,
+                      "members": [m for r in parts for m in r["members"]], "runs": len(parts)}
     out = {}
     if "fp64" not in runs:
         return {"error": "no FP64 seed run in " + str(d)}
     ref = runs["fp64"]
     for prec, r in runs.items():
-        rec = {"seeds": r["seeds"], "steps": r["steps"], "avg_from": r["avg_from"], "mlups": r["mlups"],
+        rec = {"seeds": r["seeds"], "runs": r["runs"], "steps": r["steps"], "avg_from": r["avg_from"],
+               "mlups": r["mlups"],
                "mass_drift_per_cell_mean": float(np.mean([m["mass_drift_per_cell"] for m in r["members"]])),
                "nonfinite_members": int(sum(1 for m in r["members"] if m["nonfinite"] > 0))}
         for k in ("nu_bulk", "nu_top", "nu_bottom"):
@@ -624,6 +636,8 @@ def main() -> int:
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
     ap.add_argument("--seeds", type=int, default=16, help="precision-seeds workload: members (seeds 1..N)")
+    ap.add_argument("--seed-offset", type=int, default=0,
+                    help="precision-seeds workload: seeds OFFSET+1..OFFSET+N (the report pools the runs)")
     ap.add_argument("--shape", type=int, nargs=3, default=None, metavar=("LX", "LY", "LZ"),
                     help="precision workload: cavity shape (default: a --size cube)")
     ap.add_argument("--seed", type=int, default=3, help="precision workload: seed of the initial perturbation")
@@ -663,7 +677,8 @@ def main() -> int:
     if args.workload == "precision-seeds":
         res = run_precision_seeds_workload(args)
         Path(args.out_dir).mkdir(parents=True, exist_ok=True)
-        (Path(args.out_dir) / f"precision_seeds_{args.precision}_{args.size}_{args.ra:g}.json").write_text(
+        suffix = f"_from{args.seed_offset + 1}" if args.seed_offset else ""
+        (Path(args.out_dir) / f"precision_seeds_{args.precision}_{args.size}_{args.ra:g}{suffix}.json").write_text(
             json.dumps(res))
         print(json.dumps({k: v for k, v in res.items() if k != "series"}), flush=True)
         return 0

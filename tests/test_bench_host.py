@@ -97,3 +97,29 @@ def test_precision_seeds_report_differences_and_intervals(tmp_path):
     assert r["bf16"]["nu_bulk"]["diff_vs_fp64"] - r["bf16"]["nu_bulk"]["diff_ci95"] > 0.5
     # 16 seeds with sd 0.2 each: a Welch half-width near 2.04 * 0.2 * sqrt(2/16)
     assert 0.08 < r["fp16"]["nu_bulk"]["diff_ci95"] < 0.25
+
+
+def test_precision_seeds_report_pools_runs_with_seed_offsets(tmp_path):
+    """Runs of one precision with different --seed-offset are pooled; a seed
+    that appears twice is refused."""
+    import json
+
+    b = load_bench()
+
+    def run(prec, first, n, shift):
+        members = [{"seed": first + k, "nu_bulk": 8.0 + shift + 0.01 * k, "nu_top": 8.0 + shift,
+                    "nu_bottom": 8.0 + shift, "kinetic_energy": 1e-3, "mass_drift_per_cell": 1e-6,
+                    "nonfinite": 0.0, "samples": 10} for k in range(n)]
+        return {"precision": prec, "size": 16, "ra": 1e5, "seeds": n, "first_seed": first, "steps": 100,
+                "avg_from": 50, "sample": 10, "mlups": 2.0, "members": members}
+
+    (tmp_path / "precision_seeds_fp64_16_100000.json").write_text(json.dumps(run("fp64", 1, 8, 0.0)))
+    (tmp_path / "precision_seeds_fp16_16_100000.json").write_text(json.dumps(run("fp16", 1, 8, 0.1)))
+    (tmp_path / "precision_seeds_fp16_16_100000_from9.json").write_text(json.dumps(run("fp16", 9, 8, 0.1)))
+    r = b.precision_seeds_report(16, 1e5, str(tmp_path))
+    assert r["fp16"]["seeds"] == 16 and r["fp16"]["runs"] == 2 and r["fp64"]["seeds"] == 8
+    assert r["fp16"]["mlups"] == pytest.approx(2.0)
+    assert r["fp16"]["nu_top"]["diff_vs_fp64"] == pytest.approx(0.1)
+    (tmp_path / "precision_seeds_fp16_16_100000_from5.json").write_text(json.dumps(run("fp16", 5, 4, 0.1)))
+    with pytest.raises(ValueError, match="same seed"):
+        b.precision_seeds_report(16, 1e5, str(tmp_path))
