@@ -915,8 +915,9 @@ def fp16_leg(args, fp64_mlups: float, copy_gbps: float) -> dict:
             "nonfinite": d["nonfinite"],
             "note": "same workload with populations stored as FP16 deviations, FP32 arithmetic on deviations; "
                     "accuracy against FP64: time-averaged Nu biased by at most 0.11% where FP64 converges "
-                    "(steady rolls, bench.py --workload precision --shape ...), and within the +-1.4-1.7% "
-                    "resolution of a 16-seed average in chaotic 128^3 Ra=1e6 flow (--workload precision-seeds); "
+                    "(steady rolls, bench.py --workload precision --shape ...); in chaotic 64^3 Ra=1e6 flow the "
+                    "192-seed averages differ by +0.010%, -0.010%, +0.007% (bulk, top, bottom) with 95% "
+                    "intervals of +-0.20%, which exclude a 0.5% bias (--workload precision-seeds); "
                     "BF16 storage does not meet the 0.5% bar (0.4-0.6% on walls of a steady roll, no "
                     "convection in a larger one)"}
 
