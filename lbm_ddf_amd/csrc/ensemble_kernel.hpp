@@ -4,6 +4,7 @@
 #pragma once
 
 #include "aa_kernel.hpp"
+#include "tile_order.hpp"
 
 namespace lbm {
 
@@ -30,30 +31,19 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
                                                       const uint32_t *__restrict__ tmap, int parity) {
     const uint32_t ntiles = ctrl[1];
     const uint32_t cpt = ctrl[2];  // cells per thread: a tile is cpt * BX consecutive cells
-    // Tile order.  Plain: block b takes tiles b, b+G, ...  XCD-contiguous:
-    // the blocks of XCD x (b % 8 == x) share the x-th eighth of the tile
-    // space, so neighbouring tiles -- which share the cache lines straddled by
-    // x-shifted loads -- run on one XCD's L2.
-    uint32_t t, t_end, t_step;
-    if constexpr ((FLAGS & kXcdSwizzle) != 0) {
-        constexpr uint32_t kXcds = 8;
-        const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
-        const uint32_t gx = gridDim.x / kXcds + (x < gridDim.x % kXcds ? 1u : 0u);
-        const uint32_t q = ntiles / kXcds, r = ntiles % kXcds;
-        const uint32_t start = x * q + (x < r ? x : r);
-        t = start + k;
-        t_end = start + q + (x < r ? 1u : 0u);
-        t_step = gx;
-    } else {
-        t = blockIdx.x;
-        t_end = ntiles;
-        t_step = gridDim.x;
-    }
+    // Tile order (tile_order.hpp): plain, or chunks of neighbouring tiles
+    // dealt to the XCDs in turn.  (One contiguous eighth of the tiles per
+    // XCD, used before, fetched as little but finished unevenly: 16-bit
+    // 64-cavity launch 1.79 ms against 1.76 ms in the plain order.)
+    const TileOrder order = tile_order((FLAGS & kXcdSwizzle) != 0, blockIdx.x, gridDim.x);
+    uint32_t kl = order.first;
+    uint64_t t = tile_of(order, kl), t_next = 0;
     // A tile's cavity comes from the tile map, loaded one tile ahead, so
     // that finding it costs no dependent chain of table reads.
-    uint32_t ci = t < t_end ? tmap[t] : 0u, ci_next = 0u;
-    for (; t < t_end; t += t_step, ci = ci_next) {
-        ci_next = t + t_step < t_end ? tmap[t + t_step] : 0u;
+    uint32_t ci = t < ntiles ? tmap[t] : 0u, ci_next = 0u;
+    for (; t < ntiles; kl += order.step, t = t_next, ci = ci_next) {
+        t_next = tile_of(order, kl + order.step);
+        ci_next = t_next < ntiles ? tmap[t_next] : 0u;
         const EnsCavity &c = cav[ci];
         if (!c.active) continue;
         const StepArgs &a = c.a[AA ? 0 : parity];
