@@ -33,8 +33,9 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
     const uint32_t cpt = ctrl[2];  // cells per thread: a tile is cpt * BX consecutive cells
     // Tile order (tile_order.hpp): plain, or chunks of neighbouring tiles
     // dealt to the XCDs in turn.  (One contiguous eighth of the tiles per
-    // XCD, used before, fetched as little but finished unevenly: 16-bit
-    // 64-cavity launch 1.79 ms against 1.76 ms in the plain order.)
+    // XCD fetches as little but finishes unevenly, because the XCDs then hold
+    // different cavities: 16-bit 64-cavity launch 1.79 ms against 1.76 ms in
+    // the plain order.)
     const TileOrder order = tile_order((FLAGS & kXcdSwizzle) != 0, blockIdx.x, gridDim.x);
     uint32_t kl = order.first;
     uint64_t t = tile_of(order, kl), t_next = 0;
