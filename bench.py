@@ -234,7 +234,9 @@ def run_ensemble_workload(args) -> dict:
     from lbm_ddf_amd import Cavity, Ensemble
     from lbm_ddf_amd.workloads import ensemble_cavities
 
-    cavs = ensemble_cavities()
+    from lbm_ddf_amd.workloads import SMALL_DRAW
+
+    cavs = ensemble_cavities(**SMALL_DRAW) if args.ensemble_draw == "small" else ensemble_cavities()
     cells = sum(c["lx"] * c["ly"] * c["lz"] for c in cavs)
     out = {"cavities": len(cavs), "cells": cells, "precisions": {}}
     ref_seq = None  # reference execution structure, FP64 (measured once)
@@ -608,6 +610,9 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--config1-steps", type=int, default=1000)
     ap.add_argument("--ensemble-precisions", default="fp64,fp16")
+    ap.add_argument("--ensemble-draw", default="config", choices=["config", "small"],
+                    help="ensemble workload: the 64-cavity draw (lateral 32-128, height 32-128) or 256 small "
+                         "cavities (lateral and height 16-32, Ra up to 1e6)")
     ap.add_argument("--ensemble-pattern", default="ab", choices=["ab", "aa"],
                     help="ensemble members (and the sequential comparison) with two buffers or one in place")
     ap.add_argument("--sequential-steps", type=int, default=0,
@@ -754,13 +759,16 @@ def main() -> int:
         main_prec = args.ensemble_precisions.split(",")[0]
         r = res["precisions"][main_prec]
         print(json.dumps({
-            "metric": "aggregate MLUPS, 64-cavity ensemble in one launch per step",
+            "metric": f"aggregate MLUPS, {res['cavities']}-cavity ensemble in one launch per step",
             "value": r["aggregate_mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.ensemble_steps,
             "warmup": 34, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64" if main_prec == "fp64" else f"{main_prec} storage, f32 arithmetic",
-            "data": "synthetic: 64 cavities, sizes/Ra/Pr drawn with seed 4, cavity k seeded 1000+k",
-            "config": {"workload": "64 Rayleigh-Benard cavities, lateral {32..128}, height {32,64,128}, "
-                                   "Ra log-U[1e4,1e7], Pr {0.71,1,4.4}", "cells": res["cells"],
+            "data": f"synthetic: {res['cavities']} cavities, sizes/Ra/Pr drawn with seed 4, cavity k seeded 1000+k",
+            "config": {"workload": (f"{res['cavities']} Rayleigh-Benard cavities, "
+                                    + ("lateral {16,24,32}, height {16,24,32}, Ra log-U[1e4,1e6]"
+                                       if args.ensemble_draw == "small"
+                                       else "lateral {32..128}, height {32,64,128}, Ra log-U[1e4,1e7]")
+                                    + ", Pr {0.71,1,4.4}"), "cells": res["cells"],
                        "pattern": args.ensemble_pattern},
             "ensemble": res,
         }), flush=True)

@@ -9,19 +9,27 @@ import math
 import numpy as np
 
 
-def ensemble_cavities(count: int = 64, draw_seed: int = 4) -> list:
+def ensemble_cavities(count: int = 64, draw_seed: int = 4, laterals=(32, 48, 64, 96, 128),
+                      heights=(32, 64, 128), log_ra=(4.0, 7.0)) -> list:
     """Cavities of the ensemble workload: lateral side (x and z) from
     {32, 48, 64, 96, 128}, height (y) from {32, 64, 128}, Ra log-uniform in
-    [1e4, 1e7], Pr from {0.71, 1.0, 4.4}; cavity k is seeded 1000 + k."""
+    [1e4, 1e7], Pr from {0.71, 1.0, 4.4}; cavity k is seeded 1000 + k.  Other
+    side, height and log10(Ra) ranges draw other ensembles with the same rules."""
     rng = np.random.default_rng(draw_seed)
     cavs = []
     for k in range(count):
-        lat = int(rng.choice([32, 48, 64, 96, 128]))
-        h = int(rng.choice([32, 64, 128]))
-        ra = float(10 ** rng.uniform(4.0, 7.0))
+        lat = int(rng.choice(list(laterals)))
+        h = int(rng.choice(list(heights)))
+        ra = float(10 ** rng.uniform(*log_ra))
         pr = float(rng.choice([0.71, 1.0, 4.4]))
         cavs.append(dict(lx=lat, ly=h, lz=lat, rayleigh=ra, prandtl=pr, seed=1000 + k))
     return cavs
+
+
+# The small-cavity draw: many cavities whose single-cavity steps are bound by
+# launch latency rather than HBM bandwidth (Ra up to 1e6, where tau stays
+# clear of 1/2 at these heights).
+SMALL_DRAW = dict(count=256, laterals=(16, 24, 32), heights=(16, 24, 32), log_ra=(4.0, 6.0))
 
 
 def serve_requests(rate: float, arrivals_s: float, draw_seed: int = 5, max_steps: int = 50000,

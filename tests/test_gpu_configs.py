@@ -9,6 +9,10 @@
   log-U[1e4, 1e7], Pr {0.71, 1, 4.4}): members against the oracle bit for
   bit (FP64, both storage patterns), and against standalone cavities in
   FP16 in both population layouts.
+* the small-cavity draw (256 cavities, lateral and height {16, 24, 32}, Ra
+  log-U[1e4, 1e6]) with one block per tile in both tile orders: members
+  against the oracle bit for bit (FP64) and against standalone cavities
+  (FP16).
 * the serving request mix (cube side log-U[32, 256], Ra log-U[1e4, 1e8], Pr
   {0.71, 1, 7}, Poisson arrivals), with a short step budget: the smallest
   requests' results against the oracle's diagnostics after the same number
@@ -19,7 +23,7 @@ import pytest
 
 import oracle
 from lbm_ddf_amd import Cavity, Ensemble, serve
-from lbm_ddf_amd.workloads import ensemble_cavities, serve_requests
+from lbm_ddf_amd.workloads import SMALL_DRAW, ensemble_cavities, serve_requests
 
 pytestmark = pytest.mark.gpu
 
@@ -82,12 +86,12 @@ def _ensemble(prec, pattern, cavs):
     return e, slots
 
 
-def _picks(cavs):
+def _picks(cavs, laterals=(32, 48, 64, 96, 128)):
     """Indices of the smallest and largest cavity and of one cavity of every
     lateral size (both population layouts in 16-bit storage)."""
     cells = [c["lx"] * c["ly"] * c["lz"] for c in cavs]
     picks = {int(np.argmin(cells)), int(np.argmax(cells))}
-    for lat in (32, 48, 64, 96, 128):
+    for lat in laterals:
         picks.add(next(k for k, c in enumerate(cavs) if c["lx"] == lat))
     return sorted(picks)
 
@@ -132,6 +136,56 @@ def test_64_cavity_draw_fp16_members_equal_standalone():
     e.set_tuning(256, 16, 4, 3, 4)
     e.step(10)
     for k in _picks(cavs):
+        c = cavs[k]
+        m = e.cavity(slots[k])
+        ref = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], precision="fp16",
+                     **WALLS)
+        ref.init_conductive(c["seed"], 1.0e-3)
+        ref.step(m.step_count)
+        a, b = m.state(), ref.state()
+        ref.close()
+        for key in a:
+            assert np.array_equal(a[key], b[key]), (k, (c["lx"], c["ly"], c["lz"]), key)
+    e.close()
+
+
+# ---------------------------------------------------------------- the small-cavity draw
+@pytest.mark.parametrize("flags", [3, 7])
+def test_small_cavity_draw_members_equal_oracle(flags):
+    cavs = ensemble_cavities(**SMALL_DRAW)
+    assert len(cavs) == 256
+    print(f"config: 256-cavity small draw, FP64, one block per tile, flags {flags}, prologue + 10 launches",
+          flush=True)
+    e, slots = _ensemble("fp64", "ab", cavs)
+    e.set_tuning(128, 0, 4, flags, 1)
+    e.step(10)
+    diags = e.run_diag(1)
+    assert all(diags[s]["nonfinite"] == 0 for s in slots)
+    for k in _picks(cavs, (16, 24, 32)):
+        c = cavs[k]
+        m = e.cavity(slots[k])
+        d = diags[slots[k]]
+        assert d["step"] == 11 and m.step_count == 12
+        o = oracle.OracleState(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], **WALLS)
+        o.init_conductive(c["seed"], 1.0e-3)
+        o.run(11)
+        od = o.diag()
+        for key in ("nu_bulk", "nu_top", "nu_bottom", "kinetic_energy"):
+            assert np.isclose(d[key], od[key], rtol=1e-11, atol=1e-14), (k, key, d[key], od[key])
+        o.run(1)
+        s = m.state()
+        for key in ("f", "g", "rho", "ux", "uy", "uz", "phi"):
+            assert np.array_equal(s[key], o[key]), (k, (c["lx"], c["ly"], c["lz"]), key)
+    e.close()
+
+
+def test_small_cavity_draw_fp16_members_equal_standalone():
+    cavs = ensemble_cavities(**SMALL_DRAW)
+    print("config: 256-cavity small draw, FP16 storage, one block per tile, prologue + 10 launches", flush=True)
+    e, slots = _ensemble("fp16", "ab", cavs)
+    e.set_tuning(256, 0, 4, 7, 2)
+    e.step(10)
+    for k in _picks(cavs, (16, 24, 32)):
         c = cavs[k]
         m = e.cavity(slots[k])
         ref = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], precision="fp16",
