@@ -326,7 +326,8 @@ def run_serve_workload(args) -> dict:
     from lbm_ddf_amd import Cavity, serve
     from lbm_ddf_amd.workloads import serve_requests
 
-    reqs = serve_requests(args.serve_rate, args.serve_arrivals)
+    reqs = serve_requests(args.serve_rate, args.serve_arrivals, sides=tuple(args.serve_sides),
+                          log_ra=tuple(args.serve_log_ra))
     cells = [r["lx"] ** 3 for r in reqs]
     log(f"serve: {len(reqs)} requests over {args.serve_arrivals:.0f} s, mean {sum(cells) / max(1, len(cells)) / 1e6:.2f} M "
         f"cells, precision {args.serve_precision}, pattern {args.serve_pattern}, "
@@ -355,7 +356,7 @@ def run_serve_workload(args) -> dict:
         c.close()
     seq = upd / tot / 1e6 if tot > 0 else None
     done = [r for r in res if r["status"] in ("converged", "budget")]
-    offered = sum(r["lx"] ** 3 * 50000 for r in reqs) / args.serve_arrivals / 1e6
+    offered = sum(r["lx"] ** 3 * r["max_steps"] for r in reqs) / args.serve_arrivals / 1e6
     return {"requests": len(reqs), "stats": st, "sequential_mlups": seq,
             "speedup_vs_sequential": (st["mlups"] / seq) if seq else None,
             "offered_mlups_at_step_budget": offered,
@@ -633,6 +634,10 @@ def main() -> int:
     ap.add_argument("--serve-pattern", default="ab", choices=["ab", "aa"],
                     help="resident cavities with two population buffers or one, updated in place")
     ap.add_argument("--serve-chunk", type=int, default=100)
+    ap.add_argument("--serve-sides", type=int, nargs=2, default=[32, 256], metavar=("MIN", "MAX"),
+                    help="serve workload: cube side log-uniform in [MIN, MAX]")
+    ap.add_argument("--serve-log-ra", type=float, nargs=2, default=[4.0, 8.0], metavar=("LO", "HI"),
+                    help="serve workload: log10(Ra) uniform in [LO, HI]")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     ap.add_argument("--serve-seq-sample", type=int, default=16)
     ap.add_argument("--serve-max-cells", type=float, default=64e6,
@@ -739,7 +744,8 @@ def main() -> int:
             "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64" if args.serve_precision == "fp64" else f"{args.serve_precision} storage, f32 arithmetic",
             "data": f"synthetic: Poisson arrivals {args.serve_rate:g}/s for {args.serve_arrivals:g} s, seed 5",
-            "config": {"workload": "RB cavities, side log-U[32,256], Ra log-U[1e4,1e8], Pr {0.71,1,7}, "
+            "config": {"workload": f"RB cavities, side log-U[{args.serve_sides[0]},{args.serve_sides[1]}], "
+                                   f"Ra log-U[1e{args.serve_log_ra[0]:g},1e{args.serve_log_ra[1]:g}], Pr {{0.71,1,7}}, "
                                    "stop |dNu|<1e-3 per 1000 steps or 50k steps, HBM budget "
                                    f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit,
                        "pattern": args.serve_pattern},

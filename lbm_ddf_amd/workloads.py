@@ -33,19 +33,20 @@ SMALL_DRAW = dict(count=256, laterals=(16, 24, 32), heights=(16, 24, 32), log_ra
 
 
 def serve_requests(rate: float, arrivals_s: float, draw_seed: int = 5, max_steps: int = 50000,
-                   nu_tol: float = 1.0e-3, window: int = 1000) -> list:
+                   nu_tol: float = 1.0e-3, window: int = 1000, sides=(32, 256), log_ra=(4.0, 8.0)) -> list:
     """Poisson arrivals at `rate` per second during `arrivals_s` seconds; cube
     side log-uniform in [32, 256], Ra log-uniform in [1e4, 1e8], Pr from
     {0.71, 1.0, 7.0}; a request stops when its running Nu moves by less than
-    nu_tol over `window` steps, or at max_steps."""
+    nu_tol over `window` steps, or at max_steps.  Other side and log10(Ra)
+    ranges draw other request streams with the same rules."""
     rng = np.random.default_rng(draw_seed)
     reqs, t = [], 0.0
     while True:
         t += float(rng.exponential(1.0 / rate))
         if t > arrivals_s:
             break
-        side = int(round(math.exp(rng.uniform(math.log(32), math.log(256)))))
-        reqs.append(dict(lx=side, ly=side, lz=side, rayleigh=float(10 ** rng.uniform(4.0, 8.0)),
+        side = int(round(math.exp(rng.uniform(math.log(sides[0]), math.log(sides[1])))))
+        reqs.append(dict(lx=side, ly=side, lz=side, rayleigh=float(10 ** rng.uniform(*log_ra)),
                          prandtl=float(rng.choice([0.71, 1.0, 7.0])), seed=5000 + len(reqs), amp=1.0e-3,
                          arrival_s=t, max_steps=max_steps, nu_tol=nu_tol, window=window))
     return reqs
