@@ -108,12 +108,13 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     }
 
     if constexpr (MODE == MODE_AA_PROLOGUE) {
-        rho = static_cast<T>(a.rho[idx]);
-        ux = static_cast<T>(a.ux[idx]);
-        uy = static_cast<T>(a.uy[idx]);
-        uz = static_cast<T>(a.uz[idx]);
-        phi = static_cast<T>(a.phi[idx]);
-        if constexpr (!EX) dsum = static_cast<T>(a.phi[idx] - static_cast<double>(tref_y));
+        rho = static_cast<T>(ld<false>(&a.rho[idx]));
+        ux = static_cast<T>(ld<false>(&a.ux[idx]));
+        uy = static_cast<T>(ld<false>(&a.uy[idx]));
+        uz = static_cast<T>(ld<false>(&a.uz[idx]));
+        const double phid = ld<false>(&a.phi[idx]);
+        phi = static_cast<T>(phid);
+        if constexpr (!EX) dsum = static_cast<T>(phid - static_cast<double>(tref_y));
     } else {
         // moments, summed in direction order (src/macrovar.cu:17-27)
         T jx = T(0), jy = T(0), jz = T(0), sg = T(0);
@@ -135,17 +136,17 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             dsum = sg;
             phi = tref_y + dsum;
         }
-        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
+        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(ld<false>(&lag[idx]));
         ux = jx;
         uy = jy + T(0.5) * fprev;
         uz = jz;
         if constexpr (MODE == MODE_AA_MOMENTS) {
-            a.rho[idx] = static_cast<double>(rho);
-            a.ux[idx] = static_cast<double>(ux);
-            a.uy[idx] = static_cast<double>(uy);
-            a.uz[idx] = static_cast<double>(uz);
-            a.phi[idx] = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
-            a.fy[idx] = static_cast<double>(fprev);
+            st<false>(&a.rho[idx], static_cast<double>(rho));
+            st<false>(&a.ux[idx], static_cast<double>(ux));
+            st<false>(&a.uy[idx], static_cast<double>(uy));
+            st<false>(&a.uz[idx], static_cast<double>(uz));
+            st<false>(&a.phi[idx], EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum));
+            st<false>(&a.fy[idx], static_cast<double>(fprev));
         }
     }
 
@@ -190,7 +191,7 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                 st<NTS>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
             }
         });
-        lag[idx] = P::enc_side(dphi);
+        st<false>(&lag[idx], P::enc_side(dphi));
     }
 
     if constexpr (REDUCE && MODE != MODE_AA_PROLOGUE) {

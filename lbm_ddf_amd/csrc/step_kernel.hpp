@@ -116,6 +116,26 @@ struct PolicyBF16 {
     }
 };
 
+// Loads and stores of device memory.  The pointers are cast to the global
+// address space: an ensemble kernel reads them from its descriptor table, so
+// the compiler cannot tell their address space and would otherwise use flat
+// instructions, which also count as scalar-memory operations and make each
+// wait for a descriptor field wait for every population load in flight.
+template <typename S>
+using global_t = __attribute__((address_space(1))) S;
+template <bool NT, typename S>
+__device__ __forceinline__ S ld(const S *p) {
+    const global_t<S> *g = (const global_t<S> *)p;
+    if constexpr (NT) return __builtin_nontemporal_load(g);
+    else return *g;
+}
+template <bool NT, typename S>
+__device__ __forceinline__ void st(S *p, S v) {
+    global_t<S> *g = (global_t<S> *)p;
+    if constexpr (NT) __builtin_nontemporal_store(v, g);
+    else *g = v;
+}
+
 // ---------------------------------------------------------------------------
 // Block reduction of the diagnostics (deterministic: fixed shuffle tree, then
 // waves summed in order by one thread per quantity).
@@ -147,7 +167,7 @@ __device__ __forceinline__ void block_reduce_diag(double (&q)[NQ], double *out) 
         const int k = threadIdx.x;
         double v = red[0][k];
         for (int w = 1; w < NW; ++w) v = (k == Q_MAXUSQ) ? fmax(v, red[w][k]) : v + red[w][k];
-        out[k] = v;
+        st<false>(&out[k], v);
     }
     __syncthreads();  // red[][] may be rewritten by the caller's next tile
 }
@@ -192,7 +212,7 @@ __device__ __forceinline__ void sum_records_block(const double *partials, int64_
     for (int64_t j = t; j < nblocks; j += kFinishThreads) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            const double v = partials[j * NQ + k];
+            const double v = ld<false>(&partials[j * NQ + k]);
             acc[k] = (k == Q_MAXUSQ) ? fmax(acc[k], v) : acc[k] + v;
         }
     }
@@ -350,16 +370,6 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <bool NT, typename S>
-__device__ __forceinline__ S ld(const S *p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT, typename S>
-__device__ __forceinline__ void st(S *p, S v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
 
 // ---------------------------------------------------------------------------
 // The step kernel.
@@ -412,12 +422,13 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             f[i] = P::dec_f(ld<NTL>(&fa[at + i * ps]));
             g[i] = EX ? P::template dec_g<i>(ld<NTL>(&ga[at + i * ps]), tref_y) : P::dec_f(ld<NTL>(&ga[at + i * ps]));
         });
-        rho = static_cast<T>(a.rho[idx]);
-        ux = static_cast<T>(a.ux[idx]);
-        uy = static_cast<T>(a.uy[idx]);
-        uz = static_cast<T>(a.uz[idx]);
-        phi = static_cast<T>(a.phi[idx]);
-        if constexpr (!EX) dsum = static_cast<T>(a.phi[idx] - static_cast<double>(tref_y));
+        rho = static_cast<T>(ld<false>(&a.rho[idx]));
+        ux = static_cast<T>(ld<false>(&a.ux[idx]));
+        uy = static_cast<T>(ld<false>(&a.uy[idx]));
+        uz = static_cast<T>(ld<false>(&a.uz[idx]));
+        const double phid = ld<false>(&a.phi[idx]);
+        phi = static_cast<T>(phid);
+        if constexpr (!EX) dsum = static_cast<T>(phid - static_cast<double>(tref_y));
     } else {
         // ---- gather (pull) with the wall rules -------------------------
         const bool xlo = ix > 0, xhi = ix < lx - 1;
@@ -425,7 +436,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         const bool zlo = iz > 0, zhi = iz < lz - 1;
         const bool wall = wall_lo || wall_hi;
         L rw_raw = L(0);
-        if (wall) rw_raw = wall_rho[wall_idx];
+        if (wall) rw_raw = ld<false>(&wall_rho[wall_idx]);
         // All 54 loads are issued before any value is used, so their
         // latencies overlap; a direction whose source lies outside the domain
         // reads the cell's own opposite population instead (half-way
@@ -512,7 +523,7 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             dsum = sg;
             phi = tref_y + dsum;
         }
-        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(lag[idx]);
+        const T fprev = static_cast<T>(a.grav_beta) * P::dec_side(ld<false>(&lag[idx]));
         ux = jx;
         uy = jy + T(0.5) * fprev;
         uz = jz;
@@ -522,12 +533,12 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
                 st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
                 st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
             });
-            a.rho[idx] = static_cast<double>(rho);
-            a.ux[idx] = static_cast<double>(ux);
-            a.uy[idx] = static_cast<double>(uy);
-            a.uz[idx] = static_cast<double>(uz);
-            a.phi[idx] = EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum);
-            a.fy[idx] = static_cast<double>(fprev);
+            st<false>(&a.rho[idx], static_cast<double>(rho));
+            st<false>(&a.ux[idx], static_cast<double>(ux));
+            st<false>(&a.uy[idx], static_cast<double>(uy));
+            st<false>(&a.uz[idx], static_cast<double>(uz));
+            st<false>(&a.phi[idx], EX ? static_cast<double>(phi) : static_cast<double>(tref_y) + static_cast<double>(dsum));
+            st<false>(&a.fy[idx], static_cast<double>(fprev));
         }
     }
 
@@ -541,8 +552,8 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
             st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
             st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
         });
-        lag[idx] = P::enc_side(dphi);
-        if (wall_lo || wall_hi) wall_rho[wall_idx] = P::enc_side(rho);
+        st<false>(&lag[idx], P::enc_side(dphi));
+        if (wall_lo || wall_hi) st<false>(&wall_rho[wall_idx], P::enc_side(rho));
     }
 
     if constexpr (REDUCE && MODE != MODE_PROLOGUE) {
