@@ -98,6 +98,10 @@ struct lbm_ensemble {
     int blocks_per_cu = 0;  // one block per tile (measured faster than any persistent grid)
     int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
     int flags = 7;        // launch variant of the plain step (0, 3, 4, 7)
+    // 16-bit A/B members whose buffers all span under 4 GiB: the plain step
+    // addresses populations by 32-bit byte offsets (kOff32; fewer vector
+    // instructions, which bound the 16-bit step)
+    bool off32 = false;
     int cpt = 1;          // cells per thread (1, 2, 4)
     int num_cus = 256;
     // graph of graph_pairs (parity 0, parity 1) launches
@@ -167,9 +171,13 @@ int upload_tables(lbm_ensemble *e) {
     ENS_HIP(hipStreamSynchronize(e->stream));
     uint32_t ncav = 0, tile = 0;
     uint64_t tiles = 0;
+    bool off32 = e->prec != LBM_PREC_FP64 && !e->aa;
     for (size_t s = 0; s < e->slots.size(); ++s) {
         lbm_ctx *c = e->slots[s];
         if (!c) continue;
+        // one f+g buffer: 54 populations of max(n, ps) elements
+        const uint64_t span = 54ull * std::max<uint64_t>(c->n, c->ps) * (e->prec == LBM_PREC_FP64 ? 8 : 2);
+        off32 = off32 && span < (1ull << 32);
         lbm::EnsCavity &d = e->cav_host[ncav];
         std::memset(&d, 0, sizeof(d));
         if (c->aa) {
@@ -180,6 +188,12 @@ int upload_tables(lbm_ensemble *e) {
             d.a[0] = lbm::ctx_args(c, c->f[c->cf], c->g[c->cg], c->f[1 - c->cf], c->g[1 - c->cg]);
             d.a[1] = lbm::ctx_args(c, c->f[1 - c->cf], c->g[1 - c->cg], c->f[c->cf], c->g[c->cg]);
         }
+        const lbm::DivMagic dx = lbm::div_magic(static_cast<uint32_t>(c->p.lx));
+        const lbm::DivMagic dy = lbm::div_magic(static_cast<uint32_t>(c->p.ly));
+        d.mlx = dx.m;
+        d.shlx = dx.sh;
+        d.mly = dy.m;
+        d.shly = dy.sh;
         d.d.lx = c->p.lx;
         d.d.ly = c->p.ly;
         d.d.lz = c->p.lz;
@@ -212,6 +226,8 @@ int upload_tables(lbm_ensemble *e) {
     if (tiles) ENS_HIP(hipMemcpy(e->tmap_dev, e->tmap_host.data(), sizeof(uint32_t) * tiles, hipMemcpyHostToDevice));
     // a graph of one block per tile holds the old tile count in its grid
     if (e->blocks_per_cu == 0 && static_cast<uint32_t>(tiles) != e->ntiles) drop_graph(e);
+    if (off32 != e->off32) drop_graph(e);  // the graph holds the other variant's kernel
+    e->off32 = off32;
     e->ncav = ncav;
     e->ntiles = static_cast<uint32_t>(tiles);
     e->ctrl_host[0] = ncav;
@@ -239,9 +255,13 @@ int prologue_all(lbm_ensemble *e) {
     return LBM_OK;
 }
 
+int launch_flags(const lbm_ensemble *e) {
+    return e->flags | (e->off32 ? lbm::kOff32 : 0);
+}
+
 int launch_one(lbm_ensemble *e, int parity, bool reduce) {
-    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, parity, reduce, e->aa, e->bx, e->flags, grid_of(e),
-                            e->stream));
+    ENS_HIP(lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, parity, reduce, e->aa, e->bx, launch_flags(e),
+                            grid_of(e), e->stream));
     return LBM_OK;
 }
 
@@ -250,10 +270,10 @@ int build_graph(lbm_ensemble *e) {
     hipGraph_t g = nullptr;
     ENS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < e->graph_pairs; ++k) {
-        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 0, false, e->aa, e->bx, e->flags,
-                                        grid_of(e), e->stream);
-        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 1, false, e->aa, e->bx, e->flags,
-                                        grid_of(e), e->stream);
+        hipError_t e0 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 0, false, e->aa, e->bx,
+                                        launch_flags(e), grid_of(e), e->stream);
+        hipError_t e1 = lbm::launch_ens(e->prec, e->cav_dev, e->ctrl_dev, e->tmap_dev, 1, false, e->aa, e->bx,
+                                        launch_flags(e), grid_of(e), e->stream);
         if (e0 != hipSuccess || e1 != hipSuccess) {
             (void)hipStreamEndCapture(e->stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -379,6 +399,9 @@ int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double
     if (!e || !p || !slot) return ctx_fail(LBM_E_INVALID, "NULL argument");
     if (p->precision != e->prec) return ctx_fail(LBM_E_INVALID, "cavity precision differs from the ensemble's");
     const bool aa = p->pattern == LBM_PATTERN_AA;
+    // the kernel divides cell indices below 2^31 (div_magic)
+    if (p->lx > 0 && p->ly > 0 && p->lz > 0 && static_cast<uint64_t>(p->lx) * p->ly * p->lz >= (1ull << 31))
+        return ctx_fail(LBM_E_INVALID, "an ensemble member must have fewer than 2^31 cells");
     bool empty = true;
     for (const lbm_ctx *m : e->slots) empty = empty && m == nullptr;
     if (!empty && aa != e->aa)

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "aa_kernel.hpp"
+#include "div_magic.hpp"
 #include "tile_order.hpp"
 
 namespace lbm {
@@ -19,7 +20,10 @@ struct EnsCavity {
     // In-place pattern: 1 when the cavity's pass at launch parity 0 is the
     // odd one (its buffer holds the exchanged layout), else 0.
     int32_t aa_odd0;
-    int32_t pad;
+    // Division of a cell index (< 2^31) by lx and by ly as a multiply and a
+    // shift (div_magic.hpp).
+    uint32_t mlx, mly;
+    uint32_t shlx, shly;
 };
 
 // One launch of the ensemble step: every resident cavity advances by one
@@ -51,7 +55,7 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
         const bool odd = AA && (parity ^ c.aa_odd0) != 0;
         // the cell update of this cavity's pass
         auto cell_update = [&](int ix, int iy, int iz, double(&qc)[NQ]) {
-            constexpr int FL = FLAGS & (kNtStore | kNtLoad);
+            constexpr int FL = FLAGS & (kNtStore | kNtLoad | kOff32);
             if constexpr (AA) {
                 if (odd) aa_cell<P, MODE_AA_ODD, REDUCE, FL>(a, ix, iy, iz, qc);
                 else aa_cell<P, MODE_AA_EVEN, REDUCE, FL>(a, ix, iy, iz, qc);
@@ -69,10 +73,11 @@ __global__ __launch_bounds__(BX) void ens_step_kernel(const EnsCavity *__restric
             const uint32_t cell = (lt * cpt + k) * BX + threadIdx.x;
             if (cell >= a.n) break;
             const uint32_t lx = static_cast<uint32_t>(a.lx), ly = static_cast<uint32_t>(a.ly);
-            const uint32_t r = cell / lx;
+            const uint32_t r = div_by(cell, c.mlx, c.shlx);
+            const uint32_t iz32 = div_by(r, c.mly, c.shly);
             const int ix = static_cast<int>(cell - r * lx);
-            const int iy = static_cast<int>(r % ly);
-            const int iz = static_cast<int>(r / ly);
+            const int iy = static_cast<int>(r - iz32 * ly);
+            const int iz = static_cast<int>(iz32);
             if constexpr (REDUCE) {
                 double qc[NQ];
                 cell_update(ix, iy, iz, qc);
@@ -94,7 +99,8 @@ hipError_t launch_fl(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t 
     return hipGetLastError();
 }
 
-// Flags of the plain launch: 0, 3 (non-temporal), 4 (XCD order), 7 (both).
+// Flags of the plain launch: 0, 3 (non-temporal), 4 (XCD order), 7 (both),
+// each also with kOff32 (16-bit A/B members whose buffers span under 4 GiB).
 // The launch that folds the diagnostics uses the XCD order only.
 template <class P, int BX, bool AA>
 hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t *tmap, int parity, bool reduce,
@@ -103,6 +109,15 @@ hipError_t launch_bx(const EnsCavity *cav, const uint32_t *ctrl, const uint32_t 
         hipLaunchKernelGGL((ens_step_kernel<P, MODE_FUSED, true, BX, kXcdSwizzle, AA>), dim3(grid), dim3(BX), 0, s,
                            cav, ctrl, tmap, parity);
         return hipGetLastError();
+    }
+    if constexpr (!P::exact && !AA) {
+        switch (flags) {
+            case kOff32 | 0: return launch_fl<P, BX, kOff32 | 0, AA>(cav, ctrl, tmap, parity, grid, s);
+            case kOff32 | 3: return launch_fl<P, BX, kOff32 | 3, AA>(cav, ctrl, tmap, parity, grid, s);
+            case kOff32 | 4: return launch_fl<P, BX, kOff32 | 4, AA>(cav, ctrl, tmap, parity, grid, s);
+            case kOff32 | 7: return launch_fl<P, BX, kOff32 | 7, AA>(cav, ctrl, tmap, parity, grid, s);
+            default: break;
+        }
     }
     switch (flags) {
         case 0: return launch_fl<P, BX, 0, AA>(cav, ctrl, tmap, parity, grid, s);

@@ -355,6 +355,8 @@ enum StepFlags : int {
     kNtStore = 1,  // non-temporal stores of the new populations (streamed out once)
     kNtLoad = 2,   // non-temporal loads of populations without an x shift
     kXcdSwizzle = 4,  // XCD-aware block order (see xcd_swizzle)
+    kOff32 = 16,   // population addresses as 32-bit byte offsets from the buffers'
+                   // starts (ensemble members whose buffers span under 4 GiB)
 };
 
 // The dispatcher hands workgroups to the 8 XCDs of an MI355X round-robin
@@ -388,6 +390,7 @@ template <class P, int MODE, bool REDUCE, int FLAGS>
 __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
+    constexpr bool O32 = (FLAGS & kOff32) != 0;
     constexpr bool EX = P::exact;
     using S = typename P::S;
     using T = typename P::T;
@@ -397,6 +400,12 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
     const uint32_t idx = static_cast<uint32_t>((static_cast<uint64_t>(iz) * ly + iy) * lx + ix);
     // this cell's slot of population 0 (see pop_layout)
     const uint64_t at = (static_cast<uint64_t>(iz) * ly + iy) * a.rs + ix;
+    // kOff32: the same slots as 32-bit byte offsets
+    constexpr uint32_t SZB = sizeof(S);
+    [[maybe_unused]] const uint32_t atb = static_cast<uint32_t>(at) * SZB;
+    [[maybe_unused]] const uint32_t psb = static_cast<uint32_t>(ps) * SZB;
+    [[maybe_unused]] const uint32_t rsb = static_cast<uint32_t>(a.rs) * SZB;
+    [[maybe_unused]] const uint32_t zsb = rsb * static_cast<uint32_t>(ly);
     const S *__restrict__ fa = static_cast<const S *>(a.fa);
     const S *__restrict__ ga = static_cast<const S *>(a.ga);
     S *__restrict__ fb = static_cast<S *>(a.fb);
@@ -450,6 +459,19 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
                 fr[0] = ld<NTL>(&fa[at]);
                 gr[0] = ld<NTL>(&ga[at]);
                 pulled[0] = true;
+            } else if constexpr (O32) {
+                const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
+                const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
+                const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
+                pulled[i] = inx && iny && inz;
+                // byte offsets (mod 2^32; the chosen one lies in the buffer)
+                const uint32_t src = atb + (static_cast<uint32_t>(i) * psb -
+                                            (static_cast<uint32_t>(cx * SZB) + static_cast<uint32_t>(cy) * rsb +
+                                             static_cast<uint32_t>(cz) * zsb));
+                const uint32_t own = atb + static_cast<uint32_t>(o) * psb;
+                const uint32_t off = pulled[i] ? src : own;
+                fr[i] = ld<NTL && cx == 0>(reinterpret_cast<const S *>(reinterpret_cast<const char *>(fa) + off));
+                gr[i] = ld<NTL && cx == 0>(reinterpret_cast<const S *>(reinterpret_cast<const char *>(ga) + off));
             } else {
                 const bool inx = cx == 0 ? true : (cx > 0 ? xlo : xhi);
                 const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
@@ -549,8 +571,15 @@ __device__ __forceinline__ void step_cell(const StepArgs &a, int ix, int iy, int
         collide_cell<P>(f, g, rho, ux, uy, uz, phi, dsum, F, a);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
-            st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
+            if constexpr (O32) {
+                const uint32_t off = atb + static_cast<uint32_t>(i) * psb;
+                st<NTS>(reinterpret_cast<S *>(reinterpret_cast<char *>(fb) + off), P::enc_f(f[i]));
+                st<NTS>(reinterpret_cast<S *>(reinterpret_cast<char *>(gb) + off),
+                        EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
+            } else {
+                st<NTS>(&fb[at + i * ps], P::enc_f(f[i]));
+                st<NTS>(&gb[at + i * ps], EX ? P::template enc_g<i>(g[i], tref_y) : P::enc_f(g[i]));
+            }
         });
         st<false>(&lag[idx], P::enc_side(dphi));
         if (wall_lo || wall_hi) st<false>(&wall_rho[wall_idx], P::enc_side(rho));

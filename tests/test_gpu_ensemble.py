@@ -287,3 +287,31 @@ def test_one_block_per_tile_across_joins_and_graph_replays():
     assert assert_member_equals(e, slots[2], CAVS[2]) == 30
     assert assert_member_equals(e, s4, CAVS[5]) == 21  # prologue + 8 + 12
     e.close()
+
+
+def test_16bit_addressing_follows_the_largest_member():
+    """16-bit A/B members are addressed by 32-bit byte offsets while every
+    member's buffer spans under 4 GiB, and by 64-bit ones otherwise: a member
+    beyond that size joining and leaving switches the launch (and its
+    captured graph) between the two, and every member stays exact.  (The
+    small members are read only at the end: reading a 16-bit member stores
+    its reference state, a rounding that a continuous run does not make.)"""
+    big = dict(shape=(352, 352, 328), rayleigh=1e5, prandtl=0.71, seed=21)  # 54 x 40.6 M cells x 2 B > 4 GiB
+    print("16-bit ensemble with a member of", big["shape"], flush=True)
+    e, slots = build("fp16", cavs=CAVS[:2])
+    e.set_tuning(128, 0, 2, 7, 1)  # 4 steps per graph replay
+    e.step(6)
+    sb = e.add(*big["shape"], seed=big["seed"], amp=AMP, rayleigh=big["rayleigh"], prandtl=big["prandtl"], **WALLS)
+    e.step(5)
+    m = e.cavity(sb)
+    k = m.step_count
+    assert k == 6  # prologue + 5
+    ref = standalone(big, k, "fp16")
+    for name in ("phi", "uy"):  # the populations would take 17 GB of host memory in FP64
+        assert np.array_equal(m.get_field(name), ref.get_field(name)), name
+    ref.close()
+    e.remove(sb)
+    e.step(7)
+    for s, cav in zip(slots, CAVS[:2]):
+        assert assert_member_equals(e, s, cav, "fp16") == 19  # prologue + 6 + 5 + 7
+    e.close()
