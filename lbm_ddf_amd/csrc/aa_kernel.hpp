@@ -42,6 +42,7 @@ template <class P, int MODE, bool REDUCE, int FLAGS>
 __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int iz, double (&q)[NQ]) {
     constexpr bool NTS = (FLAGS & kNtStore) != 0;
     constexpr bool NTL = (FLAGS & kNtLoad) != 0;
+    constexpr bool O32 = (FLAGS & kOff32) != 0;
     constexpr bool EX = P::exact;
     using S = typename P::S;
     using T = typename P::T;
@@ -54,6 +55,15 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
     // one buffer per distribution, read and written in place (no __restrict__)
     S *fs = static_cast<S *>(a.fb);
     S *gs = static_cast<S *>(a.gb);
+    // kOff32: the same slots as 32-bit byte offsets from the buffers' starts
+    constexpr uint32_t SZB = sizeof(S);
+    [[maybe_unused]] const uint32_t atb = static_cast<uint32_t>(at) * SZB;
+    [[maybe_unused]] const uint32_t psb = static_cast<uint32_t>(ps) * SZB;
+    [[maybe_unused]] const uint32_t rsb = static_cast<uint32_t>(a.rs) * SZB;
+    [[maybe_unused]] const uint32_t zsb = rsb * static_cast<uint32_t>(ly);
+    // a slot given by its byte offset
+    [[maybe_unused]] auto fslot = [&](uint32_t off) { return reinterpret_cast<S *>(reinterpret_cast<char *>(fs) + off); };
+    [[maybe_unused]] auto gslot = [&](uint32_t off) { return reinterpret_cast<S *>(reinterpret_cast<char *>(gs) + off); };
     const T t0 = static_cast<T>(a.t0);
     // 16-bit g storage: this row's reference temperature and the offsets of
     // the rows that wrote what this cell reads (ignored in FP64).  In
@@ -80,7 +90,17 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             const bool iny = cy == 0 ? true : (cy > 0 ? ylo : yhi);
             const bool inz = cz == 0 ? true : (cz > 0 ? zlo : zhi);
             streamed[i] = i != 0 && inx && iny && inz;
-            if constexpr (MODE == MODE_AA_ODD && i != 0) {
+            if constexpr (O32) {
+                // c_i in bytes (mod 2^32; the chosen slot lies in the buffer)
+                const uint32_t db = static_cast<uint32_t>(cx * static_cast<int>(SZB)) + static_cast<uint32_t>(cy) * rsb +
+                                    static_cast<uint32_t>(cz) * zsb;
+                const uint32_t own = atb + static_cast<uint32_t>(i) * psb;
+                uint32_t off = own;
+                if constexpr (MODE == MODE_AA_ODD && i != 0)
+                    off = streamed[i] ? atb + static_cast<uint32_t>(o) * psb - db : own;
+                fr[i] = ld<NTL && (MODE != MODE_AA_ODD || cx == 0)>(fslot(off));
+                gr[i] = ld<NTL && (MODE != MODE_AA_ODD || cx == 0)>(gslot(off));
+            } else if constexpr (MODE == MODE_AA_ODD && i != 0) {
                 const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
                 const uint64_t off = streamed[i] ? at + o * ps - static_cast<uint64_t>(d) : at + i * ps;
                 fr[i] = ld<NTL && cx == 0>(&fs[off]);
@@ -163,8 +183,13 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
             constexpr int j = I;
             constexpr int cx = CX[j], cy = CY[j], cz = CZ[j], o = OPP[j];
             if constexpr (j == 0) {
-                st<NTS>(&fs[at], P::enc_f(f[0]));
-                st<NTS>(&gs[at], EX ? P::template enc_g<0>(g[0], tref_y) : P::enc_f(g[0]));
+                if constexpr (O32) {
+                    st<NTS>(fslot(atb), P::enc_f(f[0]));
+                    st<NTS>(gslot(atb), EX ? P::template enc_g<0>(g[0], tref_y) : P::enc_f(g[0]));
+                } else {
+                    st<NTS>(&fs[at], P::enc_f(f[0]));
+                    st<NTS>(&gs[at], EX ? P::template enc_g<0>(g[0], tref_y) : P::enc_f(g[0]));
+                }
             } else {
                 const bool dx = cx == 0 ? true : (cx > 0 ? xhi : xlo);
                 const bool dy = cy == 0 ? true : (cy > 0 ? yhi : ylo);
@@ -182,13 +207,26 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                     }
                 }
                 // 16-bit g is stored against this (the writer's) row
-                uint64_t dst = at + o * ps;
-                if constexpr (MODE == MODE_AA_ODD) {
-                    const int64_t d = static_cast<int64_t>(cx) + static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
-                    if (inside) dst = at + j * ps + static_cast<uint64_t>(d);
+                if constexpr (O32) {
+                    const uint32_t own = atb + static_cast<uint32_t>(o) * psb;
+                    uint32_t dst = own;
+                    if constexpr (MODE == MODE_AA_ODD) {
+                        const uint32_t db = static_cast<uint32_t>(cx * static_cast<int>(SZB)) +
+                                            static_cast<uint32_t>(cy) * rsb + static_cast<uint32_t>(cz) * zsb;
+                        dst = inside ? atb + static_cast<uint32_t>(j) * psb + db : own;
+                    }
+                    st<NTS>(fslot(dst), P::enc_f(f[j]));
+                    st<NTS>(gslot(dst), EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
+                } else {
+                    uint64_t dst = at + o * ps;
+                    if constexpr (MODE == MODE_AA_ODD) {
+                        const int64_t d = static_cast<int64_t>(cx) +
+                                          static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
+                        if (inside) dst = at + j * ps + static_cast<uint64_t>(d);
+                    }
+                    st<NTS>(&fs[dst], P::enc_f(f[j]));
+                    st<NTS>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
                 }
-                st<NTS>(&fs[dst], P::enc_f(f[j]));
-                st<NTS>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
             }
         });
         st<false>(&lag[idx], P::enc_side(dphi));

@@ -98,9 +98,10 @@ struct lbm_ensemble {
     int blocks_per_cu = 0;  // one block per tile (measured faster than any persistent grid)
     int graph_pairs = 8;  // steps per graph replay = 2 * graph_pairs
     int flags = 7;        // launch variant of the plain step (0, 3, 4, 7)
-    // 16-bit A/B members whose buffers all span under 4 GiB: the plain step
+    // 16-bit members whose buffers all span under 4 GiB: the plain step
     // addresses populations by 32-bit byte offsets (kOff32; fewer vector
-    // instructions, which bound the 16-bit step)
+    // instructions, which bound the 16-bit step, and no branches around the
+    // in-place passes' address choices)
     bool off32 = false;
     int cpt = 1;          // cells per thread (1, 2, 4)
     int num_cus = 256;
@@ -171,7 +172,7 @@ int upload_tables(lbm_ensemble *e) {
     ENS_HIP(hipStreamSynchronize(e->stream));
     uint32_t ncav = 0, tile = 0;
     uint64_t tiles = 0;
-    bool off32 = e->prec != LBM_PREC_FP64 && !e->aa;
+    bool off32 = e->prec != LBM_PREC_FP64;
     for (size_t s = 0; s < e->slots.size(); ++s) {
         lbm_ctx *c = e->slots[s];
         if (!c) continue;

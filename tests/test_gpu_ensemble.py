@@ -289,29 +289,31 @@ def test_one_block_per_tile_across_joins_and_graph_replays():
     e.close()
 
 
-def test_16bit_addressing_follows_the_largest_member():
-    """16-bit A/B members are addressed by 32-bit byte offsets while every
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_16bit_addressing_follows_the_largest_member(pattern):
+    """16-bit members (A/B or in place) are addressed by 32-bit byte offsets while every
     member's buffer spans under 4 GiB, and by 64-bit ones otherwise: a member
     beyond that size joining and leaving switches the launch (and its
     captured graph) between the two, and every member stays exact.  (The
     small members are read only at the end: reading a 16-bit member stores
     its reference state, a rounding that a continuous run does not make.)"""
     big = dict(shape=(352, 352, 328), rayleigh=1e5, prandtl=0.71, seed=21)  # 54 x 40.6 M cells x 2 B > 4 GiB
-    print("16-bit ensemble with a member of", big["shape"], flush=True)
-    e, slots = build("fp16", cavs=CAVS[:2])
+    print("16-bit ensemble with a member of", big["shape"], "pattern", pattern, flush=True)
+    e, slots = build("fp16", cavs=CAVS[:2], pattern=pattern)
     e.set_tuning(128, 0, 2, 7, 1)  # 4 steps per graph replay
     e.step(6)
-    sb = e.add(*big["shape"], seed=big["seed"], amp=AMP, rayleigh=big["rayleigh"], prandtl=big["prandtl"], **WALLS)
+    sb = e.add(*big["shape"], seed=big["seed"], amp=AMP, rayleigh=big["rayleigh"], prandtl=big["prandtl"],
+               pattern=pattern, **WALLS)
     e.step(5)
     m = e.cavity(sb)
     k = m.step_count
     assert k == 6  # prologue + 5
-    ref = standalone(big, k, "fp16")
+    ref = standalone(big, k, "fp16", pattern)
     for name in ("phi", "uy"):  # the populations would take 17 GB of host memory in FP64
         assert np.array_equal(m.get_field(name), ref.get_field(name)), name
     ref.close()
     e.remove(sb)
     e.step(7)
     for s, cav in zip(slots, CAVS[:2]):
-        assert assert_member_equals(e, s, cav, "fp16") == 19  # prologue + 6 + 5 + 7
+        assert assert_member_equals(e, s, cav, "fp16", pattern) == 19  # prologue + 6 + 5 + 7
     e.close()
