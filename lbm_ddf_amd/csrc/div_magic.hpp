@@ -13,6 +13,9 @@
 
 namespace lbm {
 
+// div_by is exact for n below this (N = 31 above).
+constexpr uint64_t kDivLimit = 1ull << 31;
+
 struct DivMagic {
     uint32_t m, sh;
 };

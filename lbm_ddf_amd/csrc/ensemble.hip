@@ -400,8 +400,8 @@ int lbm_ensemble_add(lbm_ensemble *e, const lbm_params *p, uint64_t seed, double
     if (!e || !p || !slot) return ctx_fail(LBM_E_INVALID, "NULL argument");
     if (p->precision != e->prec) return ctx_fail(LBM_E_INVALID, "cavity precision differs from the ensemble's");
     const bool aa = p->pattern == LBM_PATTERN_AA;
-    // the kernel divides cell indices below 2^31 (div_magic)
-    if (p->lx > 0 && p->ly > 0 && p->lz > 0 && static_cast<uint64_t>(p->lx) * p->ly * p->lz >= (1ull << 31))
+    // the kernel divides cell indices below 2^31 (div_magic.hpp)
+    if (p->lx > 0 && p->ly > 0 && p->lz > 0 && static_cast<uint64_t>(p->lx) * p->ly * p->lz >= lbm::kDivLimit)
         return ctx_fail(LBM_E_INVALID, "an ensemble member must have fewer than 2^31 cells");
     bool empty = true;
     for (const lbm_ctx *m : e->slots) empty = empty && m == nullptr;

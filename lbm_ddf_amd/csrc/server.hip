@@ -6,6 +6,7 @@
 #include <thread>
 #include <vector>
 
+#include "div_magic.hpp"
 #include "internal.hpp"
 #include "serve_loop.hpp"
 
@@ -34,6 +35,10 @@ struct EnsembleBackend {
     }
     int need(const lbm_request &q, uint64_t *bytes) const {
         const lbm_params p = params(q);
+        // an ensemble member has fewer than 2^31 cells: larger requests are
+        // rejected at admission rather than failing the run at lbm_ensemble_add
+        if (q.lx > 0 && q.ly > 0 && q.lz > 0 && static_cast<uint64_t>(q.lx) * q.ly * q.lz >= lbm::kDivLimit)
+            return LBM_E_INVALID;
         return lbm_bytes_required(&p, bytes);
     }
     int add(const lbm_request &q, int32_t *slot) {

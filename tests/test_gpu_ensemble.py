@@ -8,7 +8,7 @@ with eager work between them, and across joins and removals.
 import numpy as np
 import pytest
 
-from lbm_ddf_amd import Cavity, Ensemble
+from lbm_ddf_amd import Cavity, Ensemble, LbmError
 
 pytestmark = pytest.mark.gpu
 
@@ -255,7 +255,6 @@ def test_member_cannot_be_stepped_or_destroyed_alone():
     """A member's buffers and parity belong to the ensemble's descriptor
     table: the calls that advance or free a context refuse a member, and the
     ensemble stays exact afterwards."""
-    from lbm_ddf_amd import LbmError
 
     e, slots = build(cavs=CAVS[:2])
     e.step(4)
@@ -316,4 +315,15 @@ def test_16bit_addressing_follows_the_largest_member(pattern):
     e.step(7)
     for s, cav in zip(slots, CAVS[:2]):
         assert assert_member_equals(e, s, cav, "fp16", pattern) == 19  # prologue + 6 + 5 + 7
+    e.close()
+
+
+def test_member_beyond_two_to_the_31_cells_is_refused_before_allocating():
+    e = Ensemble("fp16")
+    with pytest.raises(LbmError, match="2\\^31"):
+        e.add(1291, 1291, 1289, seed=1, amp=AMP, rayleigh=1e5, prandtl=0.71, pattern="aa", **WALLS)
+    s = e.add(*CAVS[0]["shape"], seed=CAVS[0]["seed"], amp=AMP, rayleigh=CAVS[0]["rayleigh"],
+              prandtl=CAVS[0]["prandtl"], pattern="aa", **WALLS)
+    e.step(3)
+    assert assert_member_equals(e, s, CAVS[0], "fp16", "aa") == 4
     e.close()

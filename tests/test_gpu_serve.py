@@ -100,6 +100,17 @@ def test_fp16_request_outside_the_codec_range_is_rejected_not_run():
     assert st["rejected"] == 1 and st["completed"] == 1
 
 
+def test_request_beyond_the_member_size_is_rejected_not_run():
+    """An ensemble member has fewer than 2^31 cells.  A larger in-place FP16
+    request fits a 250 GB budget (about 232 GB), so it is rejected at
+    admission, before anything is allocated, and the other request runs."""
+    big = req(1291, 1291, 1289, 1e5, 9)  # 2,148,351,809 cells
+    res, st = serve([big, D], precision="fp16", chunk_steps=50, pattern="aa")
+    assert res[0]["status"] == "rejected"
+    assert res[1]["status"] == "budget" and res[1]["steps"] == 300
+    assert st["rejected"] == 1 and st["completed"] == 1
+
+
 def test_in_place_serving_gives_the_same_results():
     """FP64 cavities kept in place (one population buffer) give every request
     exactly the A/B result."""
