@@ -286,7 +286,8 @@ int lbm_ensemble_destroy(lbm_ensemble *ens);
  * must match the resident members' (the first cavity of an empty ensemble
  * sets it): in-place members need half the population memory, so more of
  * them fit a budget; each member equals its standalone run of the same
- * pattern bit for bit. */
+ * pattern bit for bit.  A member has fewer than 2^31 cells (LBM_E_INVALID
+ * otherwise; lbm_serve rejects such a request at admission). */
 int lbm_ensemble_add(lbm_ensemble *ens, const lbm_params *params, uint64_t seed, double amp, int32_t *slot);
 int lbm_ensemble_remove(lbm_ensemble *ens, int32_t slot);
 int lbm_ensemble_count(const lbm_ensemble *ens, int32_t *slots, int32_t *resident, uint64_t *cells);
