@@ -535,14 +535,17 @@ SIGNATURES["lbm_serve"] = (ctypes.c_int, [ctypes.POINTER(Request), ctypes.c_int3
 
 
 def serve(requests: list, precision: str = "fp64", chunk_steps: int = 100, hbm_budget: float = 250e9,
-          time_limit_s: float = 60.0, max_resident: int = 0, block_x: int = 0, blocks_per_cu: int = 8,
+          time_limit_s: float = 60.0, max_resident: int = 0, block_x: int = 0, blocks_per_cu: int = 0,
           flags: int = 7, max_resident_cells: int = 0, cells_per_thread: int = 1, progress_s: float = 0.0,
           pattern: str = "ab"):
     """Run lbm_serve over a list of request dicts (keys of Request; defaults:
     walls +-0.5, amp 1e-3).  Returns (results list of dicts, stats dict).
     progress_s > 0 makes the loop print a progress line to stderr at that
     interval; pattern "aa" keeps every resident cavity in place (half the
-    population bytes; FP64 results equal to "ab")."""
+    population bytes; FP64 results equal to "ab").  block_x 0 keeps the
+    ensemble's default launch (one block per tile); block_x > 0 sets
+    (block_x, blocks_per_cu, flags, cells_per_thread), blocks_per_cu 0
+    meaning one block per tile."""
     lib = load()
     n = len(requests)
     arr = (Request * max(1, n))()

@@ -145,7 +145,7 @@ def test_argument_errors_of_the_other_entry_points_need_no_gpu():
 
     def run(**kw):
         o = dict(precision=0, max_resident=0, chunk_steps=50, hbm_budget=1 << 30, time_limit_s=1.0, block_x=0,
-                 blocks_per_cu=8, flags=7, cells_per_thread=1, max_resident_cells=0, progress_s=0.0, pattern=0,
+                 blocks_per_cu=0, flags=7, cells_per_thread=1, max_resident_cells=0, progress_s=0.0, pattern=0,
                  reserved=0)
         o.update(kw)
         opt = capi.ServeOptions(**o)
