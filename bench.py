@@ -931,7 +931,7 @@ def fp16_leg(args, fp64_mlups: float, copy_gbps: float) -> dict:
                     "accuracy against FP64: time-averaged Nu biased by at most 0.11% where FP64 converges "
                     "(steady rolls, bench.py --workload precision --shape ...); in chaotic 64^3 Ra=1e6 flow the "
                     "192-seed averages differ by +0.010%, -0.010%, +0.007% (bulk, top, bottom) with 95% "
-                    "intervals of +-0.20%, which exclude a 0.5% bias (--workload precision-seeds); at "
+                    "intervals of +-0.20-0.21%, which exclude a 0.5% bias (--workload precision-seeds); at "
                     "512^3 Ra=1e8 over steps 20k-40k the wall Nu differ by -0.059% and -0.057% "
                     "(--workload precision --ra 1e8 --steps 40000; the bulk Nu of that onset transient has "
                     "an interval wider than its mean); BF16 storage does not meet the 0.5% bar (0.4-0.6% on "
