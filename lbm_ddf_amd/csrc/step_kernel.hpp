@@ -372,7 +372,6 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-
 // ---------------------------------------------------------------------------
 // The step kernel.
 // ---------------------------------------------------------------------------
