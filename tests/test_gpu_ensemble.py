@@ -327,3 +327,24 @@ def test_member_beyond_two_to_the_31_cells_is_refused_before_allocating():
     e.step(3)
     assert assert_member_equals(e, s, CAVS[0], "fp16", "aa") == 4
     e.close()
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_16bit_member_just_under_4gib_is_exact(pattern):
+    """A 16-bit member whose buffer spans just under 4 GiB (54 x 39,766,464
+    planes x 2 B = 4,294,778,112 B) is addressed by 32-bit byte offsets up to
+    within 189 KB of 2^32; it must equal its standalone run."""
+    near = dict(shape=(340, 340, 344), rayleigh=1e5, prandtl=0.71, seed=22)
+    print("16-bit member of", near["shape"], "pattern", pattern, flush=True)
+    e = Ensemble("fp16")
+    s = e.add(*near["shape"], seed=near["seed"], amp=AMP, rayleigh=near["rayleigh"], prandtl=near["prandtl"],
+              pattern=pattern, **WALLS)
+    e.set_tuning(256, 0, 2, 7, 1)
+    e.step(9)
+    m = e.cavity(s)
+    assert m.step_count == 10  # prologue + 9
+    ref = standalone(near, 10, "fp16", pattern)
+    for name in ("phi", "uy", "rho"):
+        assert np.array_equal(m.get_field(name), ref.get_field(name)), name
+    ref.close()
+    e.close()
