@@ -129,17 +129,20 @@ def test_64_cavity_draw_members_equal_oracle(pattern):
     e.close()
 
 
-def test_64_cavity_draw_fp16_members_equal_standalone():
+@pytest.mark.parametrize("pattern,tuning", [("ab", (256, 16, 4, 3, 4)), ("aa", (256, 0, 4, 7, 1))])
+def test_64_cavity_draw_fp16_members_equal_standalone(pattern, tuning):
+    """A/B on a persistent grid with 4 cells per thread; in place with one
+    block per tile (the 16-bit members addressed by 32-bit offsets)."""
     cavs = ensemble_cavities()
-    print("config: 64-cavity draw, FP16 storage, prologue + 10 launches", flush=True)
-    e, slots = _ensemble("fp16", "ab", cavs)
-    e.set_tuning(256, 16, 4, 3, 4)
+    print(f"config: 64-cavity draw, FP16 storage, pattern {pattern}, prologue + 10 launches", flush=True)
+    e, slots = _ensemble("fp16", pattern, cavs)
+    e.set_tuning(*tuning)
     e.step(10)
     for k in _picks(cavs):
         c = cavs[k]
         m = e.cavity(slots[k])
         ref = Cavity(c["lx"], c["ly"], c["lz"], rayleigh=c["rayleigh"], prandtl=c["prandtl"], precision="fp16",
-                     **WALLS)
+                     pattern=pattern, **WALLS)
         ref.init_conductive(c["seed"], 1.0e-3)
         ref.step(m.step_count)
         a, b = m.state(), ref.state()
