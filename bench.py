@@ -332,9 +332,14 @@ def run_serve_workload(args) -> dict:
     log(f"serve: {len(reqs)} requests over {args.serve_arrivals:.0f} s, mean {sum(cells) / max(1, len(cells)) / 1e6:.2f} M "
         f"cells, precision {args.serve_precision}, pattern {args.serve_pattern}, "
         f"time limit {args.serve_time_limit:.0f} s")
+    # launch shape of the loop's ensemble: its default (one block per tile), or --serve-tuning
+    bx, bpc, fl = args.serve_tuning if args.serve_tuning else (0, 0, 7)
+    if args.serve_tuning:
+        log(f"serve: launch block_x {bx}, {bpc} blocks/CU (0: one block per tile), flags {fl}")
     res, st = serve(reqs, precision=args.serve_precision, chunk_steps=args.serve_chunk,
                     hbm_budget=args.serve_budget_gb * 1e9, time_limit_s=args.serve_time_limit,
-                    max_resident_cells=int(args.serve_max_cells), progress_s=30.0, pattern=args.serve_pattern)
+                    max_resident_cells=int(args.serve_max_cells), progress_s=30.0, pattern=args.serve_pattern,
+                    block_x=bx, blocks_per_cu=bpc, flags=fl)
     log(f"serve: {st['arrived']} arrived, {st['completed']} completed, {st['unfinished']} unfinished, "
         f"{st['pending']} pending, {st['diverged']} diverged, {st['rejected']} rejected; {st['mlups']:.0f} MLUPS; "
         f"completed p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s; all arrived (censored at the end) "
@@ -642,6 +647,9 @@ def main() -> int:
     ap.add_argument("--serve-seq-sample", type=int, default=16)
     ap.add_argument("--serve-max-cells", type=float, default=64e6,
                     help="resident-cells cap of the admission (0: HBM budget only)")
+    ap.add_argument("--serve-tuning", type=int, nargs=3, default=None, metavar=("BLOCK_X", "BLOCKS_PER_CU", "FLAGS"),
+                    help="serve workload: the ensemble's launch shape (blocks per CU 0: one block per tile); "
+                         "default: the ensemble's own"),
     ap.add_argument("--sample", type=int, default=100, help="precision workload: steps between Nu samples")
     ap.add_argument("--avg-from", type=int, default=20000, help="precision workload: average Nu after this step")
     ap.add_argument("--snapshot-stride", type=int, default=4)
@@ -748,7 +756,9 @@ def main() -> int:
                                    f"Ra log-U[1e{args.serve_log_ra[0]:g},1e{args.serve_log_ra[1]:g}], Pr {{0.71,1,7}}, "
                                    "stop |dNu|<1e-3 per 1000 steps or 50k steps, HBM budget "
                                    f"{args.serve_budget_gb:g} GB", "time_limit_s": args.serve_time_limit,
-                       "pattern": args.serve_pattern},
+                       "pattern": args.serve_pattern,
+                       "launch": (dict(zip(("block_x", "blocks_per_cu", "flags"), args.serve_tuning))
+                                  if args.serve_tuning else "ensemble default (block 128, one block per tile, flags 7)")},
             "p50_s": st["p50_s"], "p99_s": st["p99_s"],
             "p50_all_arrived_s": st["p50_all_s"], "p99_all_arrived_s": st["p99_all_s"],
             "latency_note": "p50_s/p99_s: completed requests only; *_all_arrived_s: every arrived request, "
