@@ -206,6 +206,11 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                         else gv = -gv + w2 * (twc + tw * rw);
                     }
                 }
+                // Non-temporal hint only for stores without an x shift: an
+                // odd pass's x-shifted store leaves part lines at wave edges,
+                // which the L2 completes with the neighbouring wave's part if
+                // it keeps them (a streamed part line goes to memory alone).
+                constexpr bool NTSJ = NTS && (MODE != MODE_AA_ODD || cx == 0);
                 // 16-bit g is stored against this (the writer's) row
                 if constexpr (O32) {
                     const uint32_t own = atb + static_cast<uint32_t>(o) * psb;
@@ -215,8 +220,8 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                                             static_cast<uint32_t>(cy) * rsb + static_cast<uint32_t>(cz) * zsb;
                         dst = inside ? atb + static_cast<uint32_t>(j) * psb + db : own;
                     }
-                    st<NTS>(fslot(dst), P::enc_f(f[j]));
-                    st<NTS>(gslot(dst), EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
+                    st<NTSJ>(fslot(dst), P::enc_f(f[j]));
+                    st<NTSJ>(gslot(dst), EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
                 } else {
                     uint64_t dst = at + o * ps;
                     if constexpr (MODE == MODE_AA_ODD) {
@@ -224,8 +229,8 @@ __device__ __forceinline__ void aa_cell(const StepArgs &a, int ix, int iy, int i
                                           static_cast<int64_t>(cy + cz * ly) * static_cast<int64_t>(a.rs);
                         if (inside) dst = at + j * ps + static_cast<uint64_t>(d);
                     }
-                    st<NTS>(&fs[dst], P::enc_f(f[j]));
-                    st<NTS>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
+                    st<NTSJ>(&fs[dst], P::enc_f(f[j]));
+                    st<NTSJ>(&gs[dst], EX ? P::template enc_g<j>(gv, tref_y) : P::enc_f(gv));
                 }
             }
         });
